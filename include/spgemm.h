@@ -1,0 +1,245 @@
+/*
+ * spgemm.h - C-ABI of the MI355X outer-product SpGEMM (libspgemm_amd.so).
+ *
+ * Drop-in boundary for the reference fast-squid/spGEMM hot path.  Every
+ * declaration names the reference interface it replaces.  The reference's
+ * boundary is lib/mm.h:17-25 (GPU phases), lib/coo.h:35-42 (COO container
+ * and loaders) and lib/cm.h:23-31 (compressed-matrix container), all
+ * extern "C".
+ *
+ * Differences from the reference, all deliberate:
+ *  - GPU phases take an explicit context (no file-scope device globals,
+ *    src/mm.cu:19-41) and return an int status instead of calling exit(-1)
+ *    (src/mm.cu:6-14).
+ *  - C is returned to the caller (the reference never hands it back,
+ *    src/mm.cu:152-192,420-449) with 64-bit row offsets (nnz(C) and the
+ *    intermediate count can exceed 2^31-1, src/mm.cu:27-28,173-180).
+ *  - Values are real FP32 sums (src/mm.cu:33,368,386 truncated to int).
+ *
+ * No HIP or torch types appear in the signatures: streams are void*, device
+ * arrays are plain pointers.
+ */
+#ifndef SPGEMM_AMD_H_
+#define SPGEMM_AMD_H_
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* lib/coo.h:13-14 */
+#define ROW_MAJOR 0
+#define COL_MAJOR 1
+
+/* lib/coo.h:16-22: 12-byte AoS triplet, same layout. */
+typedef struct triplet {
+    int ridx;
+    int cidx;
+    float val;
+} triplet;
+
+/* lib/coo.h:24-33: same layout. */
+typedef struct coo {
+    triplet* data;
+    int nnz;
+    int num_cols;
+    int num_rows;
+    bool type;
+} coo;
+
+/* lib/cm.h:10-21: same layout (int32 ptr, for the operands). */
+typedef struct CompressedMatrix {
+    int* ptr;
+    int* idx;
+    float* val;
+    int nnz;
+    int num_rows;
+    int num_cols;
+    bool type;
+} cm;
+
+/* Result matrix C in CSR with 64-bit row offsets (replaces the int32
+ * c_ptr_base of src/mm.cu:28).  Host arrays owned by the caller, released
+ * with spgFreeCSR64. */
+typedef struct spg_csr64 {
+    int64_t* ptr;
+    int32_t* idx;
+    float* val;
+    int64_t nnz;
+    int32_t num_rows;
+    int32_t num_cols;
+} spg_csr64;
+
+/* Status codes (the reference has none: src/mm.cu:6-14 exits). */
+#define SPG_OK 0
+#define SPG_EINVAL 1          /* bad argument or shape mismatch            */
+#define SPG_ENOMEM 2          /* device or host allocation failed          */
+#define SPG_EHIP 3            /* HIP runtime error                         */
+#define SPG_EORDER 4          /* phase called before its prerequisites     */
+#define SPG_EIO 5             /* file could not be read or parsed          */
+#define SPG_ENODEV 6          /* no usable GPU                             */
+#define SPG_EOVERFLOW 7       /* a size exceeds a representable range      */
+
+/* Operand slots. */
+#define SPG_OPERAND_A 0   /* left operand, held as CSC (COL_MAJOR)  */
+#define SPG_OPERAND_B 1   /* right operand, held as CSR (ROW_MAJOR) */
+
+/* Number of k-bins of the categorize phase: bin 0 = heavy
+ * (a_len*b_len > 50*floor(nprod/K)), bins 1..7 = b_len in [2^(b-1), 2^b),
+ * bin 8 = b_len >= 128.  Every k with a non-empty B row is binned. */
+#define SPG_NUM_KBINS 9
+
+/* Phase timer slots for spgGetPhaseTimes. */
+#define SPG_PHASE_INIT 0
+#define SPG_PHASE_CATEGORIZE 1
+#define SPG_PHASE_EXPAND 2
+#define SPG_PHASE_SYMBOLIC 3
+#define SPG_PHASE_MERGE 4
+#define SPG_NUM_PHASES 5
+
+/* readMTX flags. */
+#define SPG_MTX_COMPAT_VALUES 1  /* every value := 1.1 (src/coo.cpp:151-168) */
+
+/* ---------------- host containers (lib/coo.h:35-42, lib/cm.h:23-31) ------ */
+
+/* lib/coo.h:35 / src/coo.cpp:6-13 */
+void initCOO(coo* mat);
+/* lib/coo.h:36 / src/coo.cpp:15-22 (releases with the matching
+ * allocator, defect D13 fixed) */
+void freeCOO(coo* mat);
+/* lib/coo.h:37 / src/coo.cpp:38-41 */
+void setCOOtype(coo* mat, bool type);
+/* lib/coo.h:38 / src/coo.cpp:24-36 (also copies type, D13) */
+coo copyCOO(coo mat);
+/* lib/coo.h:39 / src/coo.cpp:73-79: sort by (major, minor, value); major
+ * is the row for ROW_MAJOR, the column for COL_MAJOR. */
+void sortCOO(coo mat);
+/* lib/coo.h:41 / src/coo.cpp:82-108: "dim n nnz" header then triplets. */
+void readCOO(coo* mat, char* path);
+/* lib/coo.h:42 / src/coo.cpp:110-173, with defect D11 fixed: Matrix
+ * Market size line read as rows cols nnz, values kept, pattern files,
+ * symmetric / skew-symmetric / hermitian expansion, any line length.  On
+ * error mat is left empty (nnz 0). */
+void readMTX(coo* mat, char* path);
+/* readMTX with flags (SPG_MTX_COMPAT_VALUES) and a status. */
+int spgReadMTX(coo* mat, const char* path, int flags);
+/* Writes C as a Matrix Market "coordinate real general" file. */
+int spgWriteMTX(const spg_csr64* C, const char* path);
+
+/* lib/cm.h:23-31 / src/cm.cpp */
+void cmInit(cm* mat);
+void cmFree(cm* mat);
+int cmGetNumRows(cm mat);
+int cmGetNumCols(cm mat);
+int cmGetNNZ(cm mat);
+void cmSetNumRows(cm* mat, int num_rows);
+void cmSetNumCols(cm* mat, int num_cols);
+void cmSetNNZ(cm* mat, int nnz);
+void cmSetType(cm* mat, bool type);
+void spgFreeCSR64(spg_csr64* mat);
+
+/* ---------------- seeded synthetic inputs (no reference counterpart) ---- */
+
+/* n_rows x n_cols, exactly nnz_per_row distinct columns per row drawn
+ * uniformly without replacement, values uniform in [-1, 1]. */
+int spgGenUniformRows(coo* out, int n_rows, int n_cols, int nnz_per_row,
+                      uint64_t seed);
+/* R-MAT with 2^row_bits rows and 2^col_bits columns, n_draws edges drawn
+ * with quadrant probabilities (a, b, c, 1-a-b-c), duplicates merged, values
+ * uniform in [-1, 1]; rows/cols >= clip_rows/clip_cols are rejected when
+ * clip_* > 0 (for non power-of-two sizes). */
+int spgGenRMAT(coo* out, int row_bits, int col_bits, int64_t n_draws,
+               double a, double b, double c, int clip_rows, int clip_cols,
+               uint64_t seed);
+/* 27-point stencil on an nx*ny*nz grid: diagonal 26, off-diagonals
+ * -1 + uniform[-0.01, 0.01]. */
+int spgGenStencil27(coo* out, int nx, int ny, int nz, uint64_t seed);
+
+/* ---------------- GPU context ------------------------------------------- */
+
+typedef struct spg_context spg_context;
+
+/* Tunables (the reference's compile-time DENSE_NUM/NUM_BINS/heavy factor,
+ * src/mm.cu:17,205,262, become runtime settings). */
+typedef struct spg_config {
+    int32_t tile_log2;         /* column tile width of long rows = 2^tile_log2 */
+    int32_t item_products;     /* target products per expansion work item      */
+    int64_t long_row_ub;       /* rows with more products are column-tiled     */
+    int32_t xcd_swizzle;       /* 1: map consecutive work items to one XCD     */
+    int32_t timing;            /* 1: record per-phase HIP events               */
+} spg_config;
+
+int spgCreate(spg_context** ctx, int device);
+int spgDestroy(spg_context* ctx);
+/* All work of the context goes to this stream (hipStream_t as void*). */
+int spgSetStream(spg_context* ctx, void* stream);
+int spgGetConfig(spg_context* ctx, spg_config* cfg);
+int spgSetConfig(spg_context* ctx, const spg_config* cfg);
+const char* spgGetErrorString(int status);
+/* Message of the last failure seen by this context. */
+const char* spgGetLastError(spg_context* ctx);
+int spgSynchronize(spg_context* ctx);
+int spgDeviceCount(int* count);
+
+/* lib/mm.h:17 cudaCOO2CM / src/mm.cu:65-136.  Uploads a COO sorted by
+ * sortCOO.  mat.type COL_MAJOR fills operand A as CSC (idx = row),
+ * ROW_MAJOR fills operand B as CSR (idx = col); the prefix sum runs on the
+ * GPU.  If out is not NULL it receives host copies (free with cmFree). */
+int spgCOO2CM(spg_context* ctx, coo mat, cm* out);
+/* Upload an already compressed host matrix into an operand slot
+ * (A: CSC, B: CSR). */
+int spgSetOperand(spg_context* ctx, int which, const cm* mat);
+/* Borrow device arrays as an operand (no copy; caller keeps ownership and
+ * keeps them alive until the context is destroyed or the slot is reset). */
+int spgSetOperandDevice(spg_context* ctx, int which, const int* d_ptr,
+                        const int* d_idx, const float* d_val, int num_rows,
+                        int num_cols, int nnz);
+
+/* lib/mm.h:18 cudaInitGEMM / src/mm.cu:138-193.  Upper bound of every C
+ * row (sum of nnz(B row k) over A(i,k)), long-row column tiling, slot
+ * bases and the intermediate buffers.  C (may be NULL) receives the
+ * shape, type ROW_MAJOR and nnz = nprod clamped to INT_MAX as the
+ * reference reports it; nprod (may be NULL) receives the exact count. */
+int spgInitGEMM(spg_context* ctx, cm* C, int64_t* nprod);
+/* lib/mm.h:19 cudaCategorizeGEMM / src/mm.cu:196-286: k-bins and the
+ * expansion work table.  bin_counts (may be NULL) has SPG_NUM_KBINS. */
+int spgCategorizeGEMM(spg_context* ctx, int64_t* bin_counts);
+/* lib/mm.h:20 cudaSimpleGEMM / src/mm.cu:332-364: expansion with one
+ * mapping for every k (the plain method). */
+int spgSimpleGEMM(spg_context* ctx);
+/* lib/mm.h:21 cudaBinGEMM / src/mm.cu:288-327: expansion with per-bin
+ * mappings. */
+int spgBinGEMM(spg_context* ctx);
+/* Exact symbolic count (no reference counterpart): distinct columns of
+ * every C row; sizes C.  nnz (may be NULL) receives nnz(C). */
+int spgSymbolicGEMM(spg_context* ctx, int64_t* nnz);
+/* lib/mm.h:25 cudaMergeGEMM / src/mm.cu:366-450: merges the intermediate
+ * products of every row into sorted CSR on the GPU (no per-row copies). */
+int spgMergeGEMM(spg_context* ctx);
+/* Copies C to the host (one bulk copy per array). */
+int spgGetC(spg_context* ctx, spg_csr64* out);
+/* Device pointers of C (valid until the next product on the context). */
+int spgGetCDevice(spg_context* ctx, const int64_t** ptr, const int32_t** idx,
+                  const float** val, int64_t* nnz);
+/* Runs InitGEMM, CategorizeGEMM, BinGEMM, SymbolicGEMM, MergeGEMM. */
+int spgMultiply(spg_context* ctx);
+
+/* Inspection for tests and tools. */
+int spgGetUpperBound(spg_context* ctx, int64_t* ub);
+int spgGetKBins(spg_context* ctx, int32_t* bin_of_k, int32_t* perm,
+                int64_t* offsets);
+int spgGetRowCounts(spg_context* ctx, int64_t* cnt);
+/* Intermediate products regrouped per C row: row_ptr (m+1 entries),
+ * cols/vals (nprod entries) in an unspecified order inside a row. */
+int spgGetIntermediateByRow(spg_context* ctx, int64_t* row_ptr,
+                            int32_t* cols, float* vals);
+/* Milliseconds per phase (SPG_NUM_PHASES) of the last product, from HIP
+ * events on the context stream; requires cfg.timing. */
+int spgGetPhaseTimes(spg_context* ctx, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1113 @@
+// spg_device.hip - CDNA4 (gfx950) kernels of the outer-product SpGEMM path
+// and their launch wrappers.
+//
+// Layout in HBM (all per context, grow-only):
+//   A  CSC  a_ptr[K+1] a_idx[nnzA] (row)  a_val[nnzA]
+//   B  CSR  b_ptr[K+1] b_idx[nnzB] (col)  b_val[nnzB]
+//   virtual rows: a C row i with more than long_row_ub products is split
+//   into T column tiles of width W = 2^tile_log2; every other row is one
+//   virtual row.  vstart[m+1] maps rows to their first virtual row, vrow_row
+//   maps back, vub/vbase give each virtual row's slot range.
+//   intermediates  icol[nprod] int32, ival[nprod] f32 (SoA), grouped by
+//   virtual row (so a C row's products are contiguous too)
+//   C  CSR  crow[m+1] int64, ccol[nnzC] int32, cval[nnzC] f32
+//
+// Value contract: a product is the FP32 product a*b (rounded once, as
+// src/mm.cu:308 computes it); the products of one C entry are summed in
+// FP64 and the sum is rounded to FP32 once.  Structure is the exact pattern
+// (numerical zeros kept), columns ascending.
+#include "spg_internal.h"
+
+#include <algorithm>
+
+namespace spg {
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+    return (1ull << lane_id()) - 1ull;
+}
+
+// Exclusive scan over a 256-thread workgroup.  sh needs kBlock/64 slots.
+template <typename T>
+__device__ __forceinline__ T block_exscan(T x, T* total, T* sh) {
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    T inc = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        T y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) sh[w] = inc;
+    __syncthreads();
+    T wpre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; ++i) {
+        T s = sh[i];
+        if (i < w) wpre += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return wpre + inc - x;
+}
+
+// Map blockIdx.x so that consecutive logical blocks run on one XCD (the
+// dispatcher places block b on XCD b % 8).  Bijective for any grid size.
+__device__ __forceinline__ long long xcd_swizzle(long long bid, long long nb) {
+    const long long per = nb / 8, rem = nb % 8;
+    const long long x = bid % 8, q = bid / 8;
+    return x < rem ? x * (per + 1) + q : rem * (per + 1) + (x - rem) * per + q;
+}
+
+template <typename T>
+__global__ void k_fill(T* p, T v, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <typename T>
+int fill(spg_context* c, T* p, T v, long long n) {
+    if (n <= 0) return SPG_OK;
+    long long nb = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)nb), dim3(kBlock), 0, c->stream, p, v, n);
+    SPG_HIP(hipGetLastError());
+    return SPG_OK;
+}
+
+// ------------------------------------------------------------------ scans
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(kBlock) void k_scan_block(const TI* __restrict__ in,
+                                                        TO* __restrict__ out,
+                                                        TO* __restrict__ bsum,
+                                                        long long n) {
+    __shared__ TO tile[kScanTile];
+    __shared__ TO sh[kBlock / 64];
+    const long long base = (long long)blockIdx.x * kScanTile;
+    // coalesced load into LDS
+#pragma unroll
+    for (int r = 0; r < kScanItems; ++r) {
+        long long g = base + r * kBlock + threadIdx.x;
+        tile[r * kBlock + threadIdx.x] = g < n ? (TO)in[g] : (TO)0;
+    }
+    __syncthreads();
+    TO v[kScanItems];
+    TO s = 0;
+#pragma unroll
+    for (int r = 0; r < kScanItems; ++r) {
+        v[r] = tile[threadIdx.x * kScanItems + r];
+        s += v[r];
+    }
+    TO tot;
+    TO pre = block_exscan<TO>(s, &tot, sh);
+#pragma unroll
+    for (int r = 0; r < kScanItems; ++r) {
+        tile[threadIdx.x * kScanItems + r] = pre;
+        pre += v[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kScanItems; ++r) {
+        long long g = base + r * kBlock + threadIdx.x;
+        if (g < n) out[g] = tile[r * kBlock + threadIdx.x];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+template <typename TO>
+__global__ __launch_bounds__(kBlock) void k_scan_add(TO* __restrict__ out,
+                                                      const TO* __restrict__ boff,
+                                                      long long n) {
+    const TO o = boff[blockIdx.x];
+    const long long base = (long long)blockIdx.x * kScanTile;
+#pragma unroll
+    for (int r = 0; r < kScanItems; ++r) {
+        long long g = base + r * kBlock + threadIdx.x;
+        if (g < n) out[g] += o;
+    }
+}
+
+template <typename TO>
+__global__ void k_copy1(TO* dst, const TO* src) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = *src;
+}
+
+template <typename TI, typename TO>
+int scan_impl(spg_context* c, const TI* in, TO* out, long long n) {
+    if (n <= 0) return fill<TO>(c, out, (TO)0, 1);
+    const long long nb = (n + kScanTile - 1) / kScanTile;
+    TO* bsum = (TO*)scratch_alloc(c, sizeof(TO) * (size_t)(nb + 1));
+    if (!bsum) return fail(c, SPG_ENOMEM, "scan scratch");
+    hipLaunchKernelGGL((k_scan_block<TI, TO>), dim3((unsigned)nb), dim3(kBlock), 0,
+                       c->stream, in, out, bsum, n);
+    SPG_HIP(hipGetLastError());
+    if (nb == 1) {
+        hipLaunchKernelGGL(k_copy1<TO>, dim3(1), dim3(64), 0, c->stream, out + n, bsum);
+        SPG_HIP(hipGetLastError());
+        return SPG_OK;
+    }
+    TO* boff = (TO*)scratch_alloc(c, sizeof(TO) * (size_t)(nb + 1));
+    if (!boff) return fail(c, SPG_ENOMEM, "scan scratch");
+    SPG_TRY((scan_impl<TO, TO>(c, bsum, boff, nb)));
+    hipLaunchKernelGGL(k_scan_add<TO>, dim3((unsigned)nb), dim3(kBlock), 0, c->stream,
+                       out, boff, n);
+    SPG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_copy1<TO>, dim3(1), dim3(64), 0, c->stream, out + n, boff + nb);
+    SPG_HIP(hipGetLastError());
+    return SPG_OK;
+}
+
+int scan_i32_to_i64(spg_context* c, const int32_t* in, int64_t* out, int64_t n) {
+    return scan_impl<int32_t, long long>(c, in, (long long*)out, n);
+}
+int scan_i64(spg_context* c, const int64_t* in, int64_t* out, int64_t n) {
+    return scan_impl<long long, long long>(c, (const long long*)in, (long long*)out, n);
+}
+int scan_i32(spg_context* c, const int32_t* in, int32_t* out, int64_t n) {
+    return scan_impl<int32_t, int32_t>(c, in, out, n);
+}
+
+// ------------------------------------------------------- stable partition
+// Keys in [-1, nb), nb <= kMaxBuckets; -1 is dropped.  Each workgroup owns
+// kPartTile consecutive keys; ranks inside a wave come from ballots, so the
+// output keeps the input order inside every bucket.
+constexpr int kMaxBuckets = 16;
+constexpr int kPartIters = 8;
+constexpr int kPartTile = kBlock * kPartIters;
+
+__global__ __launch_bounds__(kBlock) void k_part_count(const int32_t* __restrict__ key,
+                                                        long long n, int nb,
+                                                        long long nblk,
+                                                        long long* __restrict__ cnt) {
+    __shared__ int sc[kMaxBuckets];
+    if (threadIdx.x < kMaxBuckets) sc[threadIdx.x] = 0;
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * kPartTile;
+    for (int r = 0; r < kPartIters; ++r) {
+        long long g = base + r * kBlock + threadIdx.x;
+        if (g < n) {
+            int k = key[g];
+            if (k >= 0 && k < nb) atomicAdd(&sc[k], 1);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < nb) cnt[(long long)threadIdx.x * nblk + blockIdx.x] = sc[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kBlock) void k_part_scatter(const int32_t* __restrict__ key,
+                                                          long long n, int nb,
+                                                          long long nblk,
+                                                          const long long* __restrict__ off,
+                                                          int32_t* __restrict__ perm) {
+    __shared__ int wc[kBlock / 64][kMaxBuckets];
+    __shared__ long long run[kMaxBuckets];
+    const int w = threadIdx.x >> 6;
+    if (threadIdx.x < kMaxBuckets)
+        run[threadIdx.x] = threadIdx.x < nb ? off[(long long)threadIdx.x * nblk + blockIdx.x] : 0;
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * kPartTile;
+    for (int r = 0; r < kPartIters; ++r) {
+        long long g = base + r * kBlock + threadIdx.x;
+        int k = g < n ? key[g] : -1;
+        if (k >= nb) k = -1;
+        int rank = 0;
+        for (int b = 0; b < nb; ++b) {
+            unsigned long long msk = __ballot(k == b);
+            if (k == b) rank = __popcll(msk & lanemask_lt());
+            if (lane_id() == 0) wc[w][b] = __popcll(msk);
+        }
+        __syncthreads();
+        if (k >= 0) {
+            long long pos = run[k] + rank;
+            for (int q = 0; q < w; ++q) pos += wc[q][k];
+            perm[pos] = (int32_t)g;
+        }
+        __syncthreads();
+        if (threadIdx.x < nb) {
+            long long s = 0;
+            for (int q = 0; q < kBlock / 64; ++q) s += wc[q][threadIdx.x];
+            run[threadIdx.x] += s;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_part_offsets(const long long* __restrict__ off, int nb, long long nblk,
+                               long long* __restrict__ offsets) {
+    int b = threadIdx.x;
+    if (b <= nb) offsets[b] = off[(long long)b * nblk];  // off has nb*nblk+1 entries
+}
+
+int stable_partition(spg_context* c, const int32_t* key, int64_t n, int nb,
+                     int32_t* perm, int64_t* offsets) {
+    if (nb > kMaxBuckets) return fail(c, SPG_EINVAL, "too many buckets");
+    const long long nblk = std::max<long long>(1, (n + kPartTile - 1) / kPartTile);
+    long long* cnt = (long long*)scratch_alloc(c, sizeof(long long) * (size_t)(nb * nblk + 1));
+    long long* off = (long long*)scratch_alloc(c, sizeof(long long) * (size_t)(nb * nblk + 1));
+    if (!cnt || !off) return fail(c, SPG_ENOMEM, "partition scratch");
+    if (n <= 0) {
+        SPG_TRY(fill<long long>(c, (long long*)offsets, 0, nb + 1));
+        return SPG_OK;
+    }
+    hipLaunchKernelGGL(k_part_count, dim3((unsigned)nblk), dim3(kBlock), 0, c->stream,
+                       key, (long long)n, nb, nblk, cnt);
+    SPG_HIP(hipGetLastError());
+    SPG_TRY((scan_impl<long long, long long>(c, cnt, off, nb * nblk)));
+    hipLaunchKernelGGL(k_part_scatter, dim3((unsigned)nblk), dim3(kBlock), 0, c->stream,
+                       key, (long long)n, nb, nblk, off, perm);
+    SPG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_part_offsets, dim3(1), dim3(64), 0, c->stream, off, nb, nblk,
+                       (long long*)offsets);
+    SPG_HIP(hipGetLastError());
+    return SPG_OK;
+}
+
+// ------------------------------------------------------------ COO -> CM
+// src/mm.cu:43-63: minor index and value copied in COO order, a histogram
+// of the major index; the prefix sum then runs on the GPU (the reference
+// copies the histogram to the host and back, src/mm.cu:100-108).
+__global__ void k_coo2cm(const triplet* __restrict__ t, int nnz, int type,
+                         int* __restrict__ cnt, int* __restrict__ idx,
+                         float* __restrict__ val) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    triplet x = t[e];
+    int major = type == ROW_MAJOR ? x.ridx : x.cidx;
+    idx[e] = type == ROW_MAJOR ? x.cidx : x.ridx;
+    val[e] = x.val;
+    atomicAdd(&cnt[major], 1);
+}
+
+int launch_coo2cm(spg_context* c, const triplet* d_trip, int nnz, int type, int nmajor,
+                  int* ptr, int* idx, float* val) {
+    int* cnt = (int*)scratch_alloc(c, sizeof(int) * ((size_t)nmajor + 1));
+    if (!cnt) return fail(c, SPG_ENOMEM, "coo2cm scratch");
+    SPG_TRY(fill<int>(c, cnt, 0, nmajor + 1));
+    if (nnz > 0) {
+        hipLaunchKernelGGL(k_coo2cm, dim3((nnz + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                           c->stream, d_trip, nnz, type, cnt, idx, val);
+        SPG_HIP(hipGetLastError());
+    }
+    return scan_i32(c, cnt, ptr, nmajor);
+}
+
+// ------------------------------------------------------------------ init
+// src/mm.cu:138-148: every A(i,k) adds nnz(B row k) to the upper bound of
+// C row i.  One wave per column k; the B row length is read once.
+__global__ __launch_bounds__(kBlock) void k_upper_bound(const int* __restrict__ a_ptr,
+                                                         const int* __restrict__ a_idx,
+                                                         const int* __restrict__ b_ptr,
+                                                         int K,
+                                                         unsigned long long* __restrict__ ub) {
+    const long long k = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    if (k >= K) return;
+    const int blen = b_ptr[k + 1] - b_ptr[k];
+    if (blen == 0) return;
+    const int e1 = a_ptr[k + 1];
+    for (int e = a_ptr[k] + lane_id(); e < e1; e += 64)
+        atomicAdd(&ub[a_idx[e]], (unsigned long long)blen);
+}
+
+__global__ void k_classify_rows(const long long* __restrict__ ub, int m, long long long_ub,
+                                int T, int* __restrict__ ntile) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) ntile[i] = (T > 1 && ub[i] > long_ub) ? T : 1;
+}
+
+__global__ void k_fill_vrows(const int* __restrict__ vstart, const long long* __restrict__ ub,
+                             int m, int* __restrict__ vrow_row, long long* __restrict__ vub) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int s = vstart[i], e = vstart[i + 1];
+    for (int v = s; v < e; ++v) {
+        vrow_row[v] = i;
+        vub[v] = (e - s == 1) ? ub[i] : 0;
+    }
+}
+
+// Split points of every B row at the tile boundaries t*W (t = 0..T).
+__global__ void k_bsplit(const int* __restrict__ b_ptr, const int* __restrict__ b_idx, int K,
+                         int T, int tile_log2, int* __restrict__ bsplit) {
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)K * (T + 1)) return;
+    const int k = (int)(q / (T + 1)), t = (int)(q - (long long)k * (T + 1));
+    int lo = b_ptr[k], hi = b_ptr[k + 1];
+    if (t == T) {
+        bsplit[q] = hi;
+        return;
+    }
+    const long long bound = (long long)t << tile_log2;
+    while (lo < hi) {
+        int mid = lo + ((hi - lo) >> 1);
+        if (b_idx[mid] < bound) lo = mid + 1;
+        else hi = mid;
+    }
+    bsplit[q] = lo;
+}
+
+// Products of every long-row tile: sum over A(i,k) of the piece of B row k
+// inside tile t.
+__global__ __launch_bounds__(kBlock) void k_tile_bound(const int* __restrict__ a_ptr,
+                                                        const int* __restrict__ a_idx,
+                                                        const int* __restrict__ bsplit,
+                                                        const int* __restrict__ vstart, int K,
+                                                        int T,
+                                                        unsigned long long* __restrict__ vub) {
+    const long long k = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    if (k >= K) return;
+    const int* bs = bsplit + k * (T + 1);
+    if (bs[T] == bs[0]) return;
+    const int e1 = a_ptr[k + 1];
+    for (int e = a_ptr[k] + lane_id(); e < e1; e += 64) {
+        const int i = a_idx[e];
+        const int s = vstart[i];
+        if (vstart[i + 1] - s == 1) continue;
+        for (int t = 0; t < T; ++t) {
+            const int len = bs[t + 1] - bs[t];
+            if (len) atomicAdd(&vub[s + t], (unsigned long long)len);
+        }
+    }
+}
+
+int launch_init(spg_context* c) {
+    const int m = c->m, K = c->K;
+    long long* ub = (long long*)c->ub.p;
+    SPG_TRY(fill<long long>(c, ub, 0, m));
+    if (K > 0) {
+        long long nb = ((long long)K * 64 + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL(k_upper_bound, dim3((unsigned)nb), dim3(kBlock), 0, c->stream,
+                           c->A.ptr, c->A.idx, c->B.ptr, K, (unsigned long long*)ub);
+        SPG_HIP(hipGetLastError());
+    }
+    if (m > 0) {
+        hipLaunchKernelGGL(k_classify_rows, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                           c->stream, ub, m, (long long)c->cfg.long_row_ub, c->T,
+                           (int*)c->ntile.p);
+        SPG_HIP(hipGetLastError());
+    }
+    return scan_i32(c, (const int32_t*)c->ntile.p, (int32_t*)c->vstart.p, m);
+}
+
+int launch_init_vrows(spg_context* c) {
+    const int m = c->m, K = c->K;
+    if (m > 0) {
+        hipLaunchKernelGGL(k_fill_vrows, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                           c->stream, (const int*)c->vstart.p, (const long long*)c->ub.p, m,
+                           (int*)c->vrow_row.p, (long long*)c->vub.p);
+        SPG_HIP(hipGetLastError());
+    }
+    if (c->have_long && K > 0) {
+        long long nq = (long long)K * (c->T + 1);
+        hipLaunchKernelGGL(k_bsplit, dim3((unsigned)((nq + kBlock - 1) / kBlock)), dim3(kBlock),
+                           0, c->stream, c->B.ptr, c->B.idx, K, c->T, c->cfg.tile_log2,
+                           (int*)c->bsplit.p);
+        SPG_HIP(hipGetLastError());
+        long long nb = ((long long)K * 64 + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL(k_tile_bound, dim3((unsigned)nb), dim3(kBlock), 0, c->stream,
+                           c->A.ptr, c->A.idx, (const int*)c->bsplit.p,
+                           (const int*)c->vstart.p, K, c->T,
+                           (unsigned long long*)c->vub.p);
+        SPG_HIP(hipGetLastError());
+    }
+    SPG_TRY(scan_i64(c, (const int64_t*)c->vub.p, (int64_t*)c->vbase.p, c->V));
+    return fill<int>(c, (int*)c->cursor.p, 0, c->V);
+}
+
+// -------------------------------------------------------------- categorize
+// src/mm.cu:196-257 with D5 (one range), D6 (b_len >= 128 -> bin 8) and D7
+// (64-bit threshold) fixed.
+__global__ void k_kbin(const int* __restrict__ a_ptr, const int* __restrict__ b_ptr, int K,
+                       long long nprod, int* __restrict__ kbin) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const long long alen = a_ptr[k + 1] - a_ptr[k];
+    const long long blen = b_ptr[k + 1] - b_ptr[k];
+    const long long thresh = (nprod / K) * 50;
+    int bin;
+    if (blen <= 0) bin = -1;
+    else if (alen * blen > thresh) bin = 0;
+    else if (blen >= 128) bin = 8;
+    else bin = 32 - __clz((int)blen);  // floor(log2 blen) + 1
+    kbin[k] = bin;
+}
+
+// Work items of k (the "workload table"): entry chunks of about P products;
+// a B row longer than P is also cut into P-sized column chunks.
+__device__ __forceinline__ long long items_of(long long alen, long long blen, int P) {
+    if (alen == 0 || blen == 0) return 0;
+    if (blen <= P) {
+        long long epi = P / blen;
+        if (epi < 1) epi = 1;
+        return (alen + epi - 1) / epi;
+    }
+    return alen * ((blen + P - 1) / P);
+}
+
+__global__ void k_kitems(const int* __restrict__ kperm, long long nk,
+                         const int* __restrict__ a_ptr, const int* __restrict__ b_ptr, int P,
+                         int* __restrict__ kitems) {
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nk) return;
+    const int k = kperm[p];
+    kitems[p] = (int)items_of(a_ptr[k + 1] - a_ptr[k], b_ptr[k + 1] - b_ptr[k], P);
+}
+
+__global__ void k_gather_bin_items(const long long* __restrict__ kbin_off,
+                                   const long long* __restrict__ kitem_off,
+                                   long long* __restrict__ out) {
+    int b = threadIdx.x;
+    if (b <= SPG_NUM_KBINS) out[b] = kitem_off[kbin_off[b]];
+}
+
+int launch_categorize(spg_context* c) {
+    const int K = c->K;
+    if (K > 0) {
+        hipLaunchKernelGGL(k_kbin, dim3((K + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream,
+                           c->A.ptr, c->B.ptr, K, (long long)c->nprod, (int*)c->kbin.p);
+        SPG_HIP(hipGetLastError());
+    }
+    SPG_TRY(stable_partition(c, (const int32_t*)c->kbin.p, K, SPG_NUM_KBINS,
+                             (int32_t*)c->kperm.p, (int64_t*)c->kbin_off.p));
+    long long* off_h = (long long*)c->pinned_small.p;
+    SPG_HIP(hipMemcpyAsync(off_h, c->kbin_off.p, sizeof(long long) * (SPG_NUM_KBINS + 1),
+                           hipMemcpyDeviceToHost, c->stream));
+    SPG_HIP(hipStreamSynchronize(c->stream));
+    for (int b = 0; b <= SPG_NUM_KBINS; ++b) c->kbin_off_h[b] = off_h[b];
+    const long long nk = c->kbin_off_h[SPG_NUM_KBINS];
+    if (nk > 0) {
+        hipLaunchKernelGGL(k_kitems, dim3((unsigned)((nk + kBlock - 1) / kBlock)), dim3(kBlock),
+                           0, c->stream, (const int*)c->kperm.p, nk, c->A.ptr, c->B.ptr,
+                           c->cfg.item_products, (int*)c->kitems.p);
+        SPG_HIP(hipGetLastError());
+    }
+    SPG_TRY(scan_i32_to_i64(c, (const int32_t*)c->kitems.p, (int64_t*)c->kitem_off.p, nk));
+    long long* dtmp = (long long*)scratch_alloc(c, sizeof(long long) * (SPG_NUM_KBINS + 1));
+    if (!dtmp) return fail(c, SPG_ENOMEM, "categorize scratch");
+    hipLaunchKernelGGL(k_gather_bin_items, dim3(1), dim3(64), 0, c->stream,
+                       (const long long*)c->kbin_off.p, (const long long*)c->kitem_off.p, dtmp);
+    SPG_HIP(hipGetLastError());
+    SPG_HIP(hipMemcpyAsync(off_h, dtmp, sizeof(long long) * (SPG_NUM_KBINS + 1),
+                           hipMemcpyDeviceToHost, c->stream));
+    SPG_HIP(hipStreamSynchronize(c->stream));
+    for (int b = 0; b <= SPG_NUM_KBINS; ++b) c->kitems_bin_h[b] = off_h[b];
+    return SPG_OK;
+}
+
+// -------------------------------------------------------------- expansion
+// src/mm.cu:288-312 (binGEMM) / 332-354 (simpleGEMM): for the A entries of
+// one column k and B row k, every product a_ik * b_kj goes to a slot of C
+// row i.  Slots are reserved per (entry, virtual row) with one atomic, as
+// the reference reserves per row (src/mm.cu:304); lanes then write runs of
+// consecutive slots.  A group of G lanes (G = next power of two >= the B row
+// piece, at most 64) serves one A entry, so short B rows do not idle lanes.
+constexpr int kTBCap = 2048;  // LDS slot bases of long-row tiles per batch
+
+struct ExpandArgs {
+    const int* a_ptr;
+    const int* a_idx;
+    const float* a_val;
+    const int* b_ptr;
+    const int* b_idx;
+    const float* b_val;
+    const int* kperm;
+    const long long* kitem_off;
+    long long p_begin, p_end;   // positions in kperm covered by this launch
+    long long item_begin;       // first work item of this launch
+    long long nitems;
+    int P;                      // products per work item
+    int simple;                 // 1: one workgroup per k, no item table
+    const int* vstart;
+    const long long* vbase;
+    int* cursor;
+    const int* bsplit;
+    int T, tile_log2;
+    int swizzle;
+    int* icol;
+    float* ival;
+};
+
+__global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs g) {
+    __shared__ long long sbase[kBlock];
+    __shared__ float sa[kBlock];
+    __shared__ int sflag[kBlock];
+    __shared__ long long tbase[kTBCap];
+    __shared__ int sbs[kTBCap / 2 + 1];
+
+    long long bid = blockIdx.x;
+    if (g.swizzle) bid = xcd_swizzle(bid, gridDim.x);
+    int k;
+    long long e_begin, ne;
+    int j0, nj;
+    int a0, b0;
+    if (g.simple) {
+        k = (int)bid;
+        a0 = g.a_ptr[k];
+        b0 = g.b_ptr[k];
+        e_begin = a0;
+        ne = g.a_ptr[k + 1] - a0;
+        j0 = 0;
+        nj = g.b_ptr[k + 1] - b0;
+        if (ne == 0 || nj == 0) return;
+    } else {
+        const long long item = g.item_begin + bid;
+        long long lo = g.p_begin, hi = g.p_end;  // largest p with kitem_off[p] <= item
+        while (hi - lo > 1) {
+            long long mid = (lo + hi) >> 1;
+            if (g.kitem_off[mid] <= item) lo = mid;
+            else hi = mid;
+        }
+        k = g.kperm[lo];
+        const long long local = item - g.kitem_off[lo];
+        a0 = g.a_ptr[k];
+        b0 = g.b_ptr[k];
+        const long long alen = g.a_ptr[k + 1] - a0;
+        const int blen = g.b_ptr[k + 1] - b0;
+        if (blen <= g.P) {
+            long long epi = g.P / blen;
+            if (epi < 1) epi = 1;
+            e_begin = a0 + local * epi;
+            ne = alen - local * epi;
+            if (ne > epi) ne = epi;
+            j0 = 0;
+            nj = blen;
+        } else {
+            const long long jch = (blen + g.P - 1) / g.P;
+            e_begin = a0 + local / jch;
+            ne = 1;
+            j0 = (int)(local % jch) * g.P;
+            nj = blen - j0;
+            if (nj > g.P) nj = g.P;
+        }
+    }
+    const int lg = g.simple ? 6 : min(6, 32 - __clz(max(nj, 1) - 1));  // ceil(log2 nj)
+    const int G = 1 << lg;
+    int NE = kBlock >> lg;
+    const int T = g.T;
+    if (T > 1) {
+        int cap = kTBCap / T;
+        if (cap < NE) NE = cap;
+        for (int t = threadIdx.x; t <= T; t += kBlock) sbs[t] = g.bsplit[(long long)k * (T + 1) + t];
+    }
+    const int jlo = b0 + j0, jhi = b0 + j0 + nj;
+
+    for (long long eb = 0; eb < ne; eb += NE) {
+        const int cnt = (int)min((long long)NE, ne - eb);
+        __syncthreads();
+        if (threadIdx.x < cnt) {
+            const long long e = e_begin + eb + threadIdx.x;
+            const int i = g.a_idx[e];
+            sa[threadIdx.x] = g.a_val[e];
+            const int vs = g.vstart[i];
+            if (g.vstart[i + 1] - vs == 1) {
+                const int off = atomicAdd(&g.cursor[vs], nj);
+                sbase[threadIdx.x] = g.vbase[vs] + off;
+                sflag[threadIdx.x] = 0;
+            } else {
+                sbase[threadIdx.x] = vs;
+                sflag[threadIdx.x] = 1;
+            }
+        }
+        __syncthreads();
+        if (T > 1) {
+            for (int pr = threadIdx.x; pr < cnt * T; pr += kBlock) {
+                const int q = pr / T, t = pr - q * T;
+                if (!sflag[q]) continue;
+                const int s = max(sbs[t], jlo), e2 = min(sbs[t + 1], jhi);
+                if (e2 > s) {
+                    const int v = (int)sbase[q] + t;
+                    const int off = atomicAdd(&g.cursor[v], e2 - s);
+                    tbase[q * T + t] = g.vbase[v] + off - s;
+                }
+            }
+            __syncthreads();
+        }
+        const int q = threadIdx.x >> lg;
+        if (q < cnt) {
+            const float a = sa[q];
+            const int fl = sflag[q];
+            const long long base = sbase[q];
+            for (int jj = threadIdx.x & (G - 1); jj < nj; jj += G) {
+                const int j = jlo + jj;
+                const int col = g.b_idx[j];
+                const float p = __fmul_rn(a, g.b_val[j]);
+                long long slot;
+                if (!fl) slot = base + jj;
+                else slot = tbase[q * T + (col >> g.tile_log2)] + j;
+                g.icol[slot] = col;
+                g.ival[slot] = p;
+            }
+        }
+    }
+}
+
+static ExpandArgs expand_args(spg_context* c) {
+    ExpandArgs g{};
+    g.a_ptr = c->A.ptr;
+    g.a_idx = c->A.idx;
+    g.a_val = c->A.val;
+    g.b_ptr = c->B.ptr;
+    g.b_idx = c->B.idx;
+    g.b_val = c->B.val;
+    g.kperm = (const int*)c->kperm.p;
+    g.kitem_off = (const long long*)c->kitem_off.p;
+    g.P = c->cfg.item_products;
+    g.vstart = (const int*)c->vstart.p;
+    g.vbase = (const long long*)c->vbase.p;
+    g.cursor = (int*)c->cursor.p;
+    g.bsplit = (const int*)c->bsplit.p;
+    g.T = c->have_long ? c->T : 1;
+    g.tile_log2 = c->cfg.tile_log2;
+    g.swizzle = c->cfg.xcd_swizzle;
+    g.icol = (int*)c->icol.p;
+    g.ival = (float*)c->ival.p;
+    return g;
+}
+
+int launch_expand_binned(spg_context* c) {
+    ExpandArgs g = expand_args(c);
+    // heavy bin last: its items are the largest, the others fill the GPU
+    static const int order[SPG_NUM_KBINS] = {1, 2, 3, 4, 5, 6, 7, 8, 0};
+    for (int oi = 0; oi < SPG_NUM_KBINS; ++oi) {
+        const int b = order[oi];
+        const long long i0 = c->kitems_bin_h[b], i1 = c->kitems_bin_h[b + 1];
+        if (i1 <= i0) continue;
+        g.p_begin = c->kbin_off_h[b];
+        g.p_end = c->kbin_off_h[b + 1];
+        g.item_begin = i0;
+        g.nitems = i1 - i0;
+        g.simple = 0;
+        if (g.nitems > 0x7fffffffLL) return fail(c, SPG_EOVERFLOW, "too many work items");
+        hipLaunchKernelGGL(k_expand, dim3((unsigned)g.nitems), dim3(kBlock), 0, c->stream, g);
+        SPG_HIP(hipGetLastError());
+    }
+    return SPG_OK;
+}
+
+int launch_expand_simple(spg_context* c) {
+    ExpandArgs g = expand_args(c);
+    g.simple = 1;
+    g.swizzle = 0;
+    if (c->K > 0) {
+        hipLaunchKernelGGL(k_expand, dim3((unsigned)c->K), dim3(kBlock), 0, c->stream, g);
+        SPG_HIP(hipGetLastError());
+    }
+    return SPG_OK;
+}
+
+// --------------------------------------------------- virtual-row geometry
+__device__ __forceinline__ void vrow_range(int v, const int* __restrict__ vrow_row,
+                                           const int* __restrict__ vstart, int n, int tile_log2,
+                                           int& lo, int& width) {
+    const int i = vrow_row[v];
+    const int s = vstart[i];
+    if (vstart[i + 1] - s == 1) {
+        lo = 0;
+        width = n;
+    } else {
+        lo = (v - s) << tile_log2;
+        width = min(1 << tile_log2, n - lo);
+    }
+}
+
+__device__ __forceinline__ unsigned hash_slot(int key, int log2h) {
+    return ((unsigned)key * 2654435761u) >> (32 - log2h);
+}
+
+// Insert key into an LDS open-addressing table; returns the slot and sets
+// fresh when the key was not present.  The table is never more than half
+// full, so the probe sequence ends.
+template <int LOG2H>
+__device__ __forceinline__ int hash_insert(int* keys, int key, bool& fresh) {
+    constexpr int H = 1 << LOG2H;
+    unsigned h = hash_slot(key, LOG2H);
+    for (;;) {
+        int old = atomicCAS(&keys[h], -1, key);
+        if (old == -1) {
+            fresh = true;
+            return (int)h;
+        }
+        if (old == key) {
+            fresh = false;
+            return (int)h;
+        }
+        h = (h + 1) & (H - 1);
+    }
+}
+
+// --------------------------------------------------------------- symbolic
+// Class of a virtual row for the symbolic pass: a bitmap when its column
+// range fits a tile, else an LDS hash sized to twice the upper bound.
+__global__ void k_vclass_symbolic(long long V, const long long* __restrict__ vub,
+                                  const int* __restrict__ vrow_row,
+                                  const int* __restrict__ vstart, int n, int tile_log2,
+                                  int* __restrict__ cls) {
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const long long u = vub[v];
+    if (u == 0) {
+        cls[v] = -1;
+        return;
+    }
+    int lo, width;
+    vrow_range((int)v, vrow_row, vstart, n, tile_log2, lo, width);
+    if (width <= (1 << tile_log2)) {
+        cls[v] = kClassDense;
+        return;
+    }
+    long long need = 2 * min(u, (long long)width);
+    int lg = kHashMinLog2;
+    while ((1LL << lg) < need) ++lg;
+    cls[v] = lg - kHashMinLog2;  // lg <= kHashMaxLog2 since u <= long_row_ub
+}
+
+template <int LOG2H>
+__global__ __launch_bounds__(kBlock) void k_symbolic_hash(const int* __restrict__ vlist,
+                                                           const long long* __restrict__ vbase,
+                                                           const int* __restrict__ icol,
+                                                           long long* __restrict__ vcnt) {
+    constexpr int H = 1 << LOG2H;
+    __shared__ int keys[H];
+    __shared__ int cnt;
+    const int v = vlist[blockIdx.x];
+    for (int h = threadIdx.x; h < H; h += kBlock) keys[h] = -1;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    const long long s1 = vbase[v + 1];
+    int mine = 0;
+    for (long long s = vbase[v] + threadIdx.x; s < s1; s += kBlock) {
+        bool fresh;
+        hash_insert<LOG2H>(keys, icol[s], fresh);
+        mine += fresh;
+    }
+    if (mine) atomicAdd(&cnt, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) vcnt[v] = cnt;
+}
+
+__global__ __launch_bounds__(kBlock) void k_symbolic_bitmap(
+    const int* __restrict__ vlist, const long long* __restrict__ vbase,
+    const int* __restrict__ icol, const int* __restrict__ vrow_row,
+    const int* __restrict__ vstart, int n, int tile_log2, long long* __restrict__ vcnt) {
+    constexpr int NW = (1 << kTileMaxLog2) / 32;
+    __shared__ unsigned bits[NW];
+    __shared__ long long sh[kBlock / 64];
+    const int v = vlist[blockIdx.x];
+    int lo, width;
+    vrow_range(v, vrow_row, vstart, n, tile_log2, lo, width);
+    const int nw = (width + 31) >> 5;
+    for (int w = threadIdx.x; w < nw; w += kBlock) bits[w] = 0u;
+    __syncthreads();
+    const long long s1 = vbase[v + 1];
+    for (long long s = vbase[v] + threadIdx.x; s < s1; s += kBlock) {
+        const int cc = icol[s] - lo;
+        atomicOr(&bits[cc >> 5], 1u << (cc & 31));
+    }
+    __syncthreads();
+    long long mine = 0;
+    for (int w = threadIdx.x; w < nw; w += kBlock) mine += __popc(bits[w]);
+    long long tot;
+    block_exscan<long long>(mine, &tot, sh);
+    if (threadIdx.x == 0) vcnt[v] = tot;
+}
+
+__global__ void k_rowptr(const int* __restrict__ vstart, const long long* __restrict__ vout,
+                         int m, long long* __restrict__ crow) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= m) crow[i] = vout[vstart[i]];
+}
+
+template <int LOG2H>
+static void launch_sym_hash(spg_context* c, const int* list, long long cnt) {
+    hipLaunchKernelGGL(k_symbolic_hash<LOG2H>, dim3((unsigned)cnt), dim3(kBlock), 0, c->stream,
+                       list, (const long long*)c->vbase.p, (const int*)c->icol.p,
+                       (long long*)c->vcnt.p);
+}
+
+int launch_symbolic(spg_context* c) {
+    const long long V = c->V;
+    SPG_TRY(fill<long long>(c, (long long*)c->vcnt.p, 0, V));
+    if (V > 0) {
+        hipLaunchKernelGGL(k_vclass_symbolic, dim3((unsigned)((V + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, c->stream, V, (const long long*)c->vub.p,
+                           (const int*)c->vrow_row.p, (const int*)c->vstart.p, c->n,
+                           c->cfg.tile_log2, (int*)c->vclass.p);
+        SPG_HIP(hipGetLastError());
+    }
+    SPG_TRY(stable_partition(c, (const int32_t*)c->vclass.p, V, kNumVClasses,
+                             (int32_t*)c->vlist.p, (int64_t*)c->vclass_off.p));
+    long long* off_h = (long long*)c->pinned_small.p;
+    SPG_HIP(hipMemcpyAsync(off_h, c->vclass_off.p, sizeof(long long) * (kNumVClasses + 1),
+                           hipMemcpyDeviceToHost, c->stream));
+    SPG_HIP(hipStreamSynchronize(c->stream));
+    long long off[kNumVClasses + 1];
+    for (int b = 0; b <= kNumVClasses; ++b) off[b] = off_h[b];
+    const int* vl = (const int*)c->vlist.p;
+    tmark(c, SPG_PHASE_SYMBOLIC, 0);
+    for (int b = 0; b < kNumVClasses; ++b) {
+        const long long cnt = off[b + 1] - off[b];
+        if (cnt <= 0) continue;
+        const int* list = vl + off[b];
+        if (b == kClassDense) {
+            hipLaunchKernelGGL(k_symbolic_bitmap, dim3((unsigned)cnt), dim3(kBlock), 0,
+                               c->stream, list, (const long long*)c->vbase.p,
+                               (const int*)c->icol.p, (const int*)c->vrow_row.p,
+                               (const int*)c->vstart.p, c->n, c->cfg.tile_log2,
+                               (long long*)c->vcnt.p);
+        } else {
+            switch (b + kHashMinLog2) {
+                case 6: launch_sym_hash<6>(c, list, cnt); break;
+                case 7: launch_sym_hash<7>(c, list, cnt); break;
+                case 8: launch_sym_hash<8>(c, list, cnt); break;
+                case 9: launch_sym_hash<9>(c, list, cnt); break;
+                case 10: launch_sym_hash<10>(c, list, cnt); break;
+                case 11: launch_sym_hash<11>(c, list, cnt); break;
+                case 12: launch_sym_hash<12>(c, list, cnt); break;
+                case 13: launch_sym_hash<13>(c, list, cnt); break;
+                default: return fail(c, SPG_EINVAL, "bad symbolic class");
+            }
+        }
+        SPG_HIP(hipGetLastError());
+    }
+    tmark(c, SPG_PHASE_SYMBOLIC, 1);
+    SPG_TRY(scan_i64(c, (const int64_t*)c->vcnt.p, (int64_t*)c->vout.p, V));
+    return SPG_OK;
+}
+
+int launch_rowptr(spg_context* c) {
+    hipLaunchKernelGGL(k_rowptr, dim3((c->m + 1 + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                       c->stream, (const int*)c->vstart.p, (const long long*)c->vout.p, c->m,
+                       (long long*)c->crow.p);
+    SPG_HIP(hipGetLastError());
+    return SPG_OK;
+}
+
+// ------------------------------------------------------------------ merge
+// src/mm.cu:366-404 with D1-D4 fixed.  Each virtual row is merged by one
+// workgroup: FP64 accumulators in LDS keyed by column (an open-addressing
+// hash, or a dense tile when the row's column range fits one), first touch
+// by occupancy, then the columns are emitted in ascending order directly at
+// their final CSR position (the symbolic pass fixed every row's offset).
+constexpr int kDenseMinCount = 128;  // denser rows of a narrow range use a tile
+
+__global__ void k_vclass_merge(long long V, const long long* __restrict__ vcnt,
+                               const int* __restrict__ vrow_row,
+                               const int* __restrict__ vstart, int n, int tile_log2,
+                               int* __restrict__ cls) {
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const long long cnt = vcnt[v];
+    if (cnt == 0) {
+        cls[v] = -1;
+        return;
+    }
+    int lo, width;
+    vrow_range((int)v, vrow_row, vstart, n, tile_log2, lo, width);
+    if (width <= (1 << tile_log2) && (cnt > kDenseMinCount || cnt > kHashMaxCount)) {
+        cls[v] = kClassDense;
+        return;
+    }
+    int lg = kHashMinLog2;
+    while ((1LL << lg) < 2 * cnt) ++lg;
+    cls[v] = lg - kHashMinLog2;
+}
+
+template <int LOG2H>
+__global__ __launch_bounds__(kBlock) void k_merge_hash(
+    const int* __restrict__ vlist, const long long* __restrict__ vbase,
+    const int* __restrict__ icol, const float* __restrict__ ival,
+    const long long* __restrict__ vout, int* __restrict__ ccol, float* __restrict__ cval) {
+    constexpr int H = 1 << LOG2H;
+    constexpr int P = H / 2;  // at most H/2 distinct keys
+    __shared__ int keys[H];
+    __shared__ double vals[H];
+    __shared__ int skey[P];
+    __shared__ double sval[P];
+    __shared__ int cnt;
+    const int v = vlist[blockIdx.x];
+    for (int h = threadIdx.x; h < H; h += kBlock) {
+        keys[h] = -1;
+        vals[h] = 0.0;
+    }
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    const long long s1 = vbase[v + 1];
+    for (long long s = vbase[v] + threadIdx.x; s < s1; s += kBlock) {
+        bool fresh;
+        const int h = hash_insert<LOG2H>(keys, icol[s], fresh);
+        unsafeAtomicAdd(&vals[h], (double)ival[s]);
+    }
+    __syncthreads();
+    for (int h = threadIdx.x; h < H; h += kBlock) {
+        if (keys[h] != -1) {
+            const int q = atomicAdd(&cnt, 1);
+            skey[q] = keys[h];
+            sval[q] = vals[h];
+        }
+    }
+    __syncthreads();
+    const int c = cnt;
+    int np = 1;
+    while (np < c) np <<= 1;
+    for (int q = c + threadIdx.x; q < np; q += kBlock) skey[q] = 0x7fffffff;
+    __syncthreads();
+    // bitonic sort of (skey, sval), np <= P
+    for (int k = 2; k <= np; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < np; i += kBlock) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const int ki = skey[i], kj = skey[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((ki > kj) == up) {
+                        skey[i] = kj;
+                        skey[ixj] = ki;
+                        const double t = sval[i];
+                        sval[i] = sval[ixj];
+                        sval[ixj] = t;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // never write past the row's slot range fixed by the symbolic pass
+    const long long o = vout[v];
+    const int lim = (int)min((long long)c, vout[v + 1] - o);
+    for (int q = threadIdx.x; q < lim; q += kBlock) {
+        ccol[o + q] = skey[q];
+        cval[o + q] = (float)sval[q];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_merge_dense(
+    const int* __restrict__ vlist, const long long* __restrict__ vbase,
+    const int* __restrict__ icol, const float* __restrict__ ival,
+    const int* __restrict__ vrow_row, const int* __restrict__ vstart, int n, int tile_log2,
+    const long long* __restrict__ vout, int* __restrict__ ccol, float* __restrict__ cval) {
+    constexpr int WMAX = 1 << kTileMaxLog2;
+    constexpr int NW = WMAX / 32;
+    __shared__ double acc[WMAX];
+    __shared__ unsigned bits[NW];
+    __shared__ int sh[kBlock / 64];
+    const int v = vlist[blockIdx.x];
+    int lo, width;
+    vrow_range(v, vrow_row, vstart, n, tile_log2, lo, width);
+    const int nw = (width + 31) >> 5;
+    for (int w = threadIdx.x; w < nw; w += kBlock) bits[w] = 0u;
+    for (int x = threadIdx.x; x < width; x += kBlock) acc[x] = 0.0;
+    __syncthreads();
+    const long long s1 = vbase[v + 1];
+    for (long long s = vbase[v] + threadIdx.x; s < s1; s += kBlock) {
+        const int cc = icol[s] - lo;
+        atomicOr(&bits[cc >> 5], 1u << (cc & 31));
+        unsafeAtomicAdd(&acc[cc], (double)ival[s]);
+    }
+    __syncthreads();
+    // NW / kBlock words per thread, contiguous, so the emit order is ascending
+    constexpr int WPT = NW / kBlock;
+    const int w0 = threadIdx.x * WPT;
+    int mine = 0;
+#pragma unroll
+    for (int q = 0; q < WPT; ++q) mine += (w0 + q < nw) ? __popc(bits[w0 + q]) : 0;
+    int tot;
+    int pre = block_exscan<int>(mine, &tot, sh);
+    long long o = vout[v] + pre;
+    const long long lim = vout[v + 1];  // slot range fixed by the symbolic pass
+#pragma unroll
+    for (int q = 0; q < WPT; ++q) {
+        const int w = w0 + q;
+        if (w >= nw) break;
+        unsigned b = bits[w];
+        while (b && o < lim) {
+            const int bit = __ffs(b) - 1;
+            b &= b - 1;
+            const int cc = (w << 5) + bit;
+            ccol[o] = lo + cc;
+            cval[o] = (float)acc[cc];
+            ++o;
+        }
+    }
+}
+
+template <int LOG2H>
+static void launch_merge_hash(spg_context* c, const int* list, long long cnt) {
+    hipLaunchKernelGGL(k_merge_hash<LOG2H>, dim3((unsigned)cnt), dim3(kBlock), 0, c->stream,
+                       list, (const long long*)c->vbase.p, (const int*)c->icol.p,
+                       (const float*)c->ival.p, (const long long*)c->vout.p, (int*)c->ccol.p,
+                       (float*)c->cval.p);
+}
+
+int launch_merge(spg_context* c) {
+    const long long V = c->V;
+    if (V > 0) {
+        hipLaunchKernelGGL(k_vclass_merge, dim3((unsigned)((V + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, c->stream, V, (const long long*)c->vcnt.p,
+                           (const int*)c->vrow_row.p, (const int*)c->vstart.p, c->n,
+                           c->cfg.tile_log2, (int*)c->vclass.p);
+        SPG_HIP(hipGetLastError());
+    }
+    SPG_TRY(stable_partition(c, (const int32_t*)c->vclass.p, V, kNumVClasses,
+                             (int32_t*)c->vlist.p, (int64_t*)c->vclass_off.p));
+    long long* off_h = (long long*)c->pinned_small.p;
+    SPG_HIP(hipMemcpyAsync(off_h, c->vclass_off.p, sizeof(long long) * (kNumVClasses + 1),
+                           hipMemcpyDeviceToHost, c->stream));
+    SPG_HIP(hipStreamSynchronize(c->stream));
+    long long off[kNumVClasses + 1];
+    for (int b = 0; b <= kNumVClasses; ++b) off[b] = off_h[b];
+    const int* vl = (const int*)c->vlist.p;
+    // dense tiles first (largest), then hash classes from large to small
+    tmark(c, SPG_PHASE_MERGE, 0);
+    for (int b = kNumVClasses - 1; b >= 0; --b) {
+        const long long cnt = off[b + 1] - off[b];
+        if (cnt <= 0) continue;
+        const int* list = vl + off[b];
+        if (b == kClassDense) {
+            hipLaunchKernelGGL(k_merge_dense, dim3((unsigned)cnt), dim3(kBlock), 0, c->stream,
+                               list, (const long long*)c->vbase.p, (const int*)c->icol.p,
+                               (const float*)c->ival.p, (const int*)c->vrow_row.p,
+                               (const int*)c->vstart.p, c->n, c->cfg.tile_log2,
+                               (const long long*)c->vout.p, (int*)c->ccol.p,
+                               (float*)c->cval.p);
+        } else {
+            switch (b + kHashMinLog2) {
+                case 6: launch_merge_hash<6>(c, list, cnt); break;
+                case 7: launch_merge_hash<7>(c, list, cnt); break;
+                case 8: launch_merge_hash<8>(c, list, cnt); break;
+                case 9: launch_merge_hash<9>(c, list, cnt); break;
+                case 10: launch_merge_hash<10>(c, list, cnt); break;
+                case 11: launch_merge_hash<11>(c, list, cnt); break;
+                case 12: launch_merge_hash<12>(c, list, cnt); break;
+                case 13: launch_merge_hash<13>(c, list, cnt); break;
+                default: return fail(c, SPG_EINVAL, "bad merge class");
+            }
+        }
+        SPG_HIP(hipGetLastError());
+    }
+    tmark(c, SPG_PHASE_MERGE, 1);
+    return SPG_OK;
+}
+
+// ------------------------------------------------ intermediate inspection
+__global__ void k_inter_rowptr(const int* __restrict__ vstart,
+                               const long long* __restrict__ vbase, int m,
+                               long long* __restrict__ rp) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= m) rp[i] = vbase[vstart[i]];
+}
+
+int launch_regroup_intermediate(spg_context* c, int64_t* d_rowptr) {
+    hipLaunchKernelGGL(k_inter_rowptr, dim3((c->m + 1 + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                       c->stream, (const int*)c->vstart.p, (const long long*)c->vbase.p, c->m,
+                       (long long*)d_rowptr);
+    SPG_HIP(hipGetLastError());
+    return SPG_OK;
+}
+
+template int fill<int>(spg_context*, int*, int, long long);
+template int fill<long long>(spg_context*, long long*, long long, long long);
+
+}  // namespace spg

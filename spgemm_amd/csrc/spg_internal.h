@@ -1,0 +1,135 @@
+// spg_internal.h - context and launch-wrapper declarations shared by the
+// HIP translation units of libspgemm_amd.so.  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/spgemm.h"
+
+namespace spg {
+
+// Upper limits of the merge structures held in LDS.
+constexpr int kBlock = 256;             // threads per workgroup
+constexpr int kHashMaxLog2 = 13;        // largest LDS hash: 8192 slots
+constexpr int kHashMaxCount = 1 << (kHashMaxLog2 - 1);  // <= 4096 keys
+constexpr int kTileMaxLog2 = 13;        // dense tile: 8192 FP64 accumulators
+constexpr int kHashMinLog2 = 6;         // smallest hash: 64 slots
+// Merge/symbolic classes of a virtual row.
+constexpr int kNumHashClasses = kHashMaxLog2 - kHashMinLog2 + 1;  // 64..8192
+constexpr int kClassDense = kNumHashClasses;                      // dense tile
+constexpr int kNumVClasses = kNumHashClasses + 1;
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+struct Operand {
+    const int* ptr = nullptr;
+    const int* idx = nullptr;
+    const float* val = nullptr;
+    int num_rows = 0, num_cols = 0, nnz = 0;
+    bool set = false;
+    bool owned = false;
+    DevBuf bptr, bidx, bval;  // storage when owned
+};
+
+struct Timer {
+    hipEvent_t ev[2 * SPG_NUM_PHASES] = {};
+    bool recorded[SPG_NUM_PHASES] = {};
+};
+
+}  // namespace spg
+
+struct spg_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    spg_config cfg{};
+    std::string err;
+
+    spg::Operand A, B;  // A: CSC (ptr over columns), B: CSR (ptr over rows)
+
+    // shape of the product
+    int32_t m = 0, K = 0, n = 0;
+    int32_t T = 1;        // column tiles of a long row
+    int32_t W = 0;        // tile width
+    int64_t V = 0;        // virtual rows
+    int64_t nprod = 0;
+    int64_t nnzC = 0;
+    bool have_long = false;
+
+    // phase state
+    int stage = 0;  // 0 none, 1 init, 2 categorize, 3 expanded, 4 symbolic, 5 merged
+
+    // init products
+    spg::DevBuf ub, ntile, vstart, vrow_row, bsplit, vub, vbase, cursor;
+    // categorize products
+    spg::DevBuf kbin, kperm, kbin_off, kitems, kitem_off;
+    int64_t kbin_off_h[SPG_NUM_KBINS + 1] = {};
+    int64_t kitems_bin_h[SPG_NUM_KBINS + 1] = {};  // item offsets at bin starts
+    // intermediates
+    spg::DevBuf icol, ival;
+    // symbolic / merge
+    spg::DevBuf vclass, vlist, vclass_off, vcnt, vout, crow, ccol, cval;
+    // scratch for scans and partitions
+    spg::DevBuf scratch;
+    size_t scratch_top = 0;
+    spg::DevBuf pinned_small;  // host-pinned staging for small readbacks
+
+    spg::Timer timer;
+    double phase_ms[SPG_NUM_PHASES] = {};
+};
+
+namespace spg {
+
+// Device memory helpers (spg_api.hip).
+int ensure(spg_context* c, DevBuf& b, size_t bytes);
+void* scratch_alloc(spg_context* c, size_t bytes);
+void scratch_reset(spg_context* c);
+int fail(spg_context* c, int code, const std::string& msg);
+// Records the start (end = 0) or end (end = 1) event of a phase timer.
+void tmark(spg_context* c, int phase, int end);
+// Fill kernel (used instead of hipMemsetAsync so the path stays capturable).
+template <typename T>
+int fill(spg_context* c, T* p, T v, long long n);
+
+// ---- launch wrappers (spg_device.hip); all enqueue on c->stream ----
+// exclusive scan: out[0..n] with out[n] = total
+int scan_i32_to_i64(spg_context* c, const int32_t* in, int64_t* out, int64_t n);
+int scan_i64(spg_context* c, const int64_t* in, int64_t* out, int64_t n);
+int scan_i32(spg_context* c, const int32_t* in, int32_t* out, int64_t n);
+// stable partition of n keys in [-1, nb) (-1 dropped) into perm; offsets
+// (nb+1 entries, device) receive bucket starts.
+int stable_partition(spg_context* c, const int32_t* key, int64_t n, int nb,
+                     int32_t* perm, int64_t* offsets);
+
+int launch_coo2cm(spg_context* c, const triplet* d_trip, int nnz, int type,
+                  int nmajor, int* ptr, int* idx, float* val);
+int launch_init(spg_context* c);
+int launch_categorize(spg_context* c);
+int launch_expand_binned(spg_context* c);
+int launch_expand_simple(spg_context* c);
+int launch_symbolic(spg_context* c);
+int launch_merge(spg_context* c);
+int launch_regroup_intermediate(spg_context* c, int64_t* d_rowptr);
+
+}  // namespace spg
+
+#define SPG_HIP(call)                                                        \
+    do {                                                                     \
+        hipError_t e_ = (call);                                              \
+        if (e_ != hipSuccess)                                                \
+            return ::spg::fail(c, SPG_EHIP,                                  \
+                               std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+#define SPG_TRY(expr)              \
+    do {                           \
+        int r_ = (expr);           \
+        if (r_ != SPG_OK) return r_; \
+    } while (0)
