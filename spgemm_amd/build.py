@@ -46,8 +46,14 @@ def build_native(verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    cli = os.path.join(LIBDIR, "spgemm_outer")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-x", "c++", os.path.join(HERE, "cli", "spgemm_outer.cpp"),
+           "-x", "none", LIB, "-Wl,-rpath,$ORIGIN", "-o", cli]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"cli build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {cli}")
     return LIB
 
 

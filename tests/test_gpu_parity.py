@@ -179,31 +179,40 @@ def test_work_item_shapes(spg, orc, gpu_ctx, item_products, swz):
 
 
 def test_edge_structures(spg, orc, gpu_ctx):
-    """Empty rows/cols, b_len = 0, b_len >= 128 (never binned by the
-    reference, D6), a heavy k (star), one dense row, duplicate entries."""
+    """Empty rows/cols (b_len = 0), b_len >= 128 (never binned by the
+    reference, D6), a heavy k, one dense row, duplicate entries."""
     print("case edge structures", flush=True)
     n = 600
+    rng = np.random.default_rng(1)
     rows, cols = [], []
-    for j in range(n):          # dense column 3 and dense row 5 (star through k=3)
-        rows += [j, 5]
-        cols += [3, j]
-    for j in range(0, n, 4):    # a B row with >= 128 entries (row 11)
+    for j in range(590):        # dense column 3 (rows 590.. stay empty)
+        rows.append(j)
+        cols.append(3)
+    for j in range(n):          # dense row 5: k = 5 has b_len 600 (bin 8)
+        rows.append(5)
+        cols.append(j)
+    for j in range(0, n, 4):    # row 11: b_len 150 (bin 8)
         rows.append(11)
         cols.append(j)
     rows += [100, 100, 100]     # duplicate entries of (100, 200)
     cols += [200, 200, 201]
+    for i in range(20, 560):    # background, 12 per row
+        for c in rng.choice(n, 12, replace=False):
+            rows.append(i)
+            cols.append(int(c))
+    for c in rng.choice(n, 60, replace=False):  # row 3: k = 3 becomes heavy
+        rows.append(3)
+        cols.append(int(c))
     rows = np.array(rows, np.int32)
     cols = np.array(cols, np.int32)
-    rng = np.random.default_rng(1)
     vals = (rng.integers(-64, 64, len(rows)) / 64.0).astype(np.float32)
     A = (rows, cols, vals)
     sh = (n, n)
-    # row/col 599 and many others are empty
     out = run_gpu(spg, gpu_ctx, A, sh, A, sh, via_coo=False)
     ref = oracle_all(orc, A, sh, A, sh)
     compare(out, ref)
     obins = ref["kbins"][0]
-    assert (obins == 0).any() and (obins == 8).any() and (obins == -1).any()
+    assert obins[3] == 0 and obins[5] == 8 and obins[11] == 8 and (obins == -1).any()
 
 
 def test_tiny_shapes(spg, orc, gpu_ctx):
