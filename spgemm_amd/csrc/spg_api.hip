@@ -151,6 +151,11 @@ int spgCreate(spg_context** out, int device) {
         rc = SPG_EHIP;
     }
     c->own_stream = rc == SPG_OK;
+    if (rc == SPG_OK) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+            c->num_cu = prop.multiProcessorCount;
+    }
     for (int i = 0; rc == SPG_OK && i < 2 * SPG_NUM_PHASES; ++i)
         if (hipEventCreate(&c->timer.ev[i]) != hipSuccess) rc = SPG_EHIP;
     if (rc == SPG_OK && hipHostMalloc(&c->pinned_small.p, 4096, hipHostMallocDefault) != hipSuccess)
@@ -171,7 +176,8 @@ int spgDestroy(spg_context* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     operand_release(c->A);
     operand_release(c->B);
-    DevBuf* bufs[] = {&c->ub, &c->ntile, &c->vstart, &c->vrow_row, &c->bsplit, &c->vub,
+    DevBuf* bufs[] = {&c->ub, &c->ntile, &c->vstart, &c->vrow_row, &c->vlo, &c->vwid,
+                      &c->counters, &c->bsplit, &c->vub,
                       &c->vbase, &c->cursor, &c->kbin, &c->kperm, &c->kbin_off, &c->kitems,
                       &c->kitem_off, &c->icol, &c->ival, &c->vclass, &c->vlist,
                       &c->vclass_off, &c->vcnt, &c->vout, &c->crow, &c->ccol, &c->cval,
@@ -373,6 +379,9 @@ int spgInitGEMM(spg_context* c, cm* C, int64_t* nprod) {
     c->V = V;
     c->have_long = c->V > c->m;
     SPG_TRY(ensure(c, c->vrow_row, sizeof(int32_t) * ((size_t)c->V + 1)));
+    SPG_TRY(ensure(c, c->vlo, sizeof(int32_t) * ((size_t)c->V + 1)));
+    SPG_TRY(ensure(c, c->vwid, sizeof(int32_t) * ((size_t)c->V + 1)));
+    SPG_TRY(ensure(c, c->counters, sizeof(uint64_t) * 64));
     SPG_TRY(ensure(c, c->vub, sizeof(int64_t) * ((size_t)c->V + 1)));
     SPG_TRY(ensure(c, c->vbase, sizeof(int64_t) * ((size_t)c->V + 1)));
     SPG_TRY(ensure(c, c->cursor, sizeof(int32_t) * ((size_t)c->V + 1)));

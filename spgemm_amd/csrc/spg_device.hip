@@ -319,13 +319,23 @@ __global__ void k_classify_rows(const long long* __restrict__ ub, int m, long lo
 }
 
 __global__ void k_fill_vrows(const int* __restrict__ vstart, const long long* __restrict__ ub,
-                             int m, int* __restrict__ vrow_row, long long* __restrict__ vub) {
+                             int m, int n, int tile_log2, int* __restrict__ vrow_row,
+                             long long* __restrict__ vub, int* __restrict__ vlo,
+                             int* __restrict__ vwid) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const int s = vstart[i], e = vstart[i + 1];
     for (int v = s; v < e; ++v) {
         vrow_row[v] = i;
-        vub[v] = (e - s == 1) ? ub[i] : 0;
+        if (e - s == 1) {
+            vub[v] = ub[i];
+            vlo[v] = 0;
+            vwid[v] = n;
+        } else {
+            const int lo = (v - s) << tile_log2;
+            vlo[v] = lo;
+            vwid[v] = min(1 << tile_log2, n - lo);
+        }
     }
 }
 
@@ -397,7 +407,8 @@ int launch_init_vrows(spg_context* c) {
     if (m > 0) {
         hipLaunchKernelGGL(k_fill_vrows, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0,
                            c->stream, (const int*)c->vstart.p, (const long long*)c->ub.p, m,
-                           (int*)c->vrow_row.p, (long long*)c->vub.p);
+                           c->n, c->cfg.tile_log2, (int*)c->vrow_row.p, (long long*)c->vub.p,
+                           (int*)c->vlo.p, (int*)c->vwid.p);
         SPG_HIP(hipGetLastError());
     }
     if (c->have_long && K > 0) {
@@ -698,21 +709,6 @@ int launch_expand_simple(spg_context* c) {
     return SPG_OK;
 }
 
-// --------------------------------------------------- virtual-row geometry
-__device__ __forceinline__ void vrow_range(int v, const int* __restrict__ vrow_row,
-                                           const int* __restrict__ vstart, int n, int tile_log2,
-                                           int& lo, int& width) {
-    const int i = vrow_row[v];
-    const int s = vstart[i];
-    if (vstart[i + 1] - s == 1) {
-        lo = 0;
-        width = n;
-    } else {
-        lo = (v - s) << tile_log2;
-        width = min(1 << tile_log2, n - lo);
-    }
-}
-
 __device__ __forceinline__ unsigned hash_slot(int key, int log2h) {
     return ((unsigned)key * 2654435761u) >> (32 - log2h);
 }
@@ -739,11 +735,21 @@ __device__ __forceinline__ int hash_insert(int* keys, int key, bool& fresh) {
 }
 
 // --------------------------------------------------------------- symbolic
-// Class of a virtual row for the symbolic pass: a bitmap when its column
-// range fits a tile, else an LDS hash sized to twice the upper bound.
+// Classes of a virtual row.  0..7: LDS hash with 2^(6+class) slots; TILE:
+// the row's column range fits one tile (bitmap / dense FP64 accumulator);
+// RANK: a row over the full column range (n <= 2^18) merged through a
+// bitmap of all columns whose prefix popcounts give each column its output
+// position (no sort).
+constexpr int kClassTile = kNumHashClasses;      // 8
+constexpr int kClassRank = kNumHashClasses + 1;  // 9
+constexpr int kRankMaxWords = 8192;              // bitmap of 2^18 columns
+constexpr int kRankMinCount = 128;               // smaller rows: hash + sort
+constexpr int kPT = 512;                          // threads of the persistent kernels
+
+static_assert(kNumVClasses >= kClassRank + 1, "class table too small");
+
 __global__ void k_vclass_symbolic(long long V, const long long* __restrict__ vub,
-                                  const int* __restrict__ vrow_row,
-                                  const int* __restrict__ vstart, int n, int tile_log2,
+                                  const int* __restrict__ vwid, int W,
                                   int* __restrict__ cls) {
     long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
@@ -752,10 +758,9 @@ __global__ void k_vclass_symbolic(long long V, const long long* __restrict__ vub
         cls[v] = -1;
         return;
     }
-    int lo, width;
-    vrow_range((int)v, vrow_row, vstart, n, tile_log2, lo, width);
-    if (width <= (1 << tile_log2)) {
-        cls[v] = kClassDense;
+    const int width = vwid[v];
+    if (width <= W) {
+        cls[v] = kClassTile;
         return;
     }
     long long need = 2 * min(u, (long long)width);
@@ -788,30 +793,78 @@ __global__ __launch_bounds__(kBlock) void k_symbolic_hash(const int* __restrict_
     if (threadIdx.x == 0) vcnt[v] = cnt;
 }
 
-__global__ __launch_bounds__(kBlock) void k_symbolic_bitmap(
-    const int* __restrict__ vlist, const long long* __restrict__ vbase,
-    const int* __restrict__ icol, const int* __restrict__ vrow_row,
-    const int* __restrict__ vstart, int n, int tile_log2, long long* __restrict__ vcnt) {
-    constexpr int NW = (1 << kTileMaxLog2) / 32;
-    __shared__ unsigned bits[NW];
-    __shared__ long long sh[kBlock / 64];
-    const int v = vlist[blockIdx.x];
-    int lo, width;
-    vrow_range(v, vrow_row, vstart, n, tile_log2, lo, width);
-    const int nw = (width + 31) >> 5;
-    for (int w = threadIdx.x; w < nw; w += kBlock) bits[w] = 0u;
+// Exclusive scan over an NT-thread workgroup (sh: NT/64 slots).
+template <typename T, int NT>
+__device__ __forceinline__ T block_exscan_n(T x, T* total, T* sh) {
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    T inc = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        T y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) sh[w] = inc;
     __syncthreads();
-    const long long s1 = vbase[v + 1];
-    for (long long s = vbase[v] + threadIdx.x; s < s1; s += kBlock) {
-        const int cc = icol[s] - lo;
-        atomicOr(&bits[cc >> 5], 1u << (cc & 31));
+    T wpre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) {
+        T s = sh[i];
+        if (i < w) wpre += s;
+        tot += s;
     }
     __syncthreads();
-    long long mine = 0;
-    for (int w = threadIdx.x; w < nw; w += kBlock) mine += __popc(bits[w]);
-    long long tot;
-    block_exscan<long long>(mine, &tot, sh);
-    if (threadIdx.x == 0) vcnt[v] = tot;
+    *total = tot;
+    return wpre + inc - x;
+}
+
+// Persistent loop helper: the workgroup takes list entries from a global
+// counter (largest rows come first in the list, so the tail is short).
+__device__ __forceinline__ long long next_item(unsigned long long* ctr, long long* sj) {
+    __syncthreads();
+    if (threadIdx.x == 0) *sj = (long long)atomicAdd(ctr, 1ull);
+    __syncthreads();
+    return *sj;
+}
+
+// Symbolic count of tile-width virtual rows: a bitmap of the tile's columns.
+// Four product loads are in flight per thread before the LDS updates.
+__global__ __launch_bounds__(kPT) void k_symbolic_tile(
+    const int* __restrict__ vlist, long long nlist, unsigned long long* __restrict__ ctr,
+    const long long* __restrict__ vbase, const int* __restrict__ vlo,
+    const int* __restrict__ vwid, const int* __restrict__ icol, long long* __restrict__ vcnt) {
+    constexpr int NW = (1 << kTileMaxLog2) / 32;
+    __shared__ unsigned bits[NW];
+    __shared__ int sh[kPT / 64];
+    __shared__ long long sj;
+    for (int w = threadIdx.x; w < NW; w += kPT) bits[w] = 0u;
+    for (;;) {
+        const long long j = next_item(ctr, &sj);
+        if (j >= nlist) break;
+        const int v = vlist[j];
+        const long long s0 = vbase[v], s1 = vbase[v + 1];
+        const int lo = vlo[v];
+        const int nw = (vwid[v] + 31) >> 5;
+        for (long long s = s0 + threadIdx.x; s < s1; s += 4 * kPT) {
+            int c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const long long q = s + (long long)u * kPT;
+                c[u] = q < s1 ? icol[q] - lo : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (c[u] >= 0) atomicOr(&bits[c[u] >> 5], 1u << (c[u] & 31));
+        }
+        __syncthreads();
+        int mine = 0;
+        for (int w = threadIdx.x; w < nw; w += kPT) {
+            mine += __popc(bits[w]);
+            bits[w] = 0u;
+        }
+        int tot;
+        block_exscan_n<int, kPT>(mine, &tot, sh);
+        if (threadIdx.x == 0) vcnt[v] = tot;
+    }
 }
 
 __global__ void k_rowptr(const int* __restrict__ vstart, const long long* __restrict__ vout,
@@ -827,36 +880,48 @@ static void launch_sym_hash(spg_context* c, const int* list, long long cnt) {
                        (long long*)c->vcnt.p);
 }
 
-int launch_symbolic(spg_context* c) {
-    const long long V = c->V;
-    SPG_TRY(fill<long long>(c, (long long*)c->vcnt.p, 0, V));
-    if (V > 0) {
-        hipLaunchKernelGGL(k_vclass_symbolic, dim3((unsigned)((V + kBlock - 1) / kBlock)),
-                           dim3(kBlock), 0, c->stream, V, (const long long*)c->vub.p,
-                           (const int*)c->vrow_row.p, (const int*)c->vstart.p, c->n,
-                           c->cfg.tile_log2, (int*)c->vclass.p);
-        SPG_HIP(hipGetLastError());
-    }
-    SPG_TRY(stable_partition(c, (const int32_t*)c->vclass.p, V, kNumVClasses,
-                             (int32_t*)c->vlist.p, (int64_t*)c->vclass_off.p));
+// Grid of a persistent kernel: every workgroup that can be resident.
+static unsigned persistent_grid(spg_context* c, long long nlist, int per_cu) {
+    long long g = (long long)c->num_cu * per_cu;
+    if (g > nlist) g = nlist;
+    return (unsigned)(g > 0 ? g : 1);
+}
+
+static int class_offsets(spg_context* c, long long* off) {
     long long* off_h = (long long*)c->pinned_small.p;
     SPG_HIP(hipMemcpyAsync(off_h, c->vclass_off.p, sizeof(long long) * (kNumVClasses + 1),
                            hipMemcpyDeviceToHost, c->stream));
     SPG_HIP(hipStreamSynchronize(c->stream));
-    long long off[kNumVClasses + 1];
     for (int b = 0; b <= kNumVClasses; ++b) off[b] = off_h[b];
+    return SPG_OK;
+}
+
+int launch_symbolic(spg_context* c) {
+    const long long V = c->V;
+    SPG_TRY(fill<long long>(c, (long long*)c->vcnt.p, 0, V));
+    SPG_TRY(fill<unsigned long long>(c, (unsigned long long*)c->counters.p, 0ull, 64));
+    if (V > 0) {
+        hipLaunchKernelGGL(k_vclass_symbolic, dim3((unsigned)((V + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, c->stream, V, (const long long*)c->vub.p,
+                           (const int*)c->vwid.p, c->W, (int*)c->vclass.p);
+        SPG_HIP(hipGetLastError());
+    }
+    SPG_TRY(stable_partition(c, (const int32_t*)c->vclass.p, V, kNumVClasses,
+                             (int32_t*)c->vlist.p, (int64_t*)c->vclass_off.p));
+    long long off[kNumVClasses + 1];
+    SPG_TRY(class_offsets(c, off));
     const int* vl = (const int*)c->vlist.p;
+    unsigned long long* ctr = (unsigned long long*)c->counters.p;
     tmark(c, SPG_PHASE_SYMBOLIC, 0);
     for (int b = 0; b < kNumVClasses; ++b) {
         const long long cnt = off[b + 1] - off[b];
         if (cnt <= 0) continue;
         const int* list = vl + off[b];
-        if (b == kClassDense) {
-            hipLaunchKernelGGL(k_symbolic_bitmap, dim3((unsigned)cnt), dim3(kBlock), 0,
-                               c->stream, list, (const long long*)c->vbase.p,
-                               (const int*)c->icol.p, (const int*)c->vrow_row.p,
-                               (const int*)c->vstart.p, c->n, c->cfg.tile_log2,
-                               (long long*)c->vcnt.p);
+        if (b == kClassTile) {
+            hipLaunchKernelGGL(k_symbolic_tile, dim3(persistent_grid(c, cnt, 4)), dim3(kPT), 0,
+                               c->stream, list, cnt, ctr + 0, (const long long*)c->vbase.p,
+                               (const int*)c->vlo.p, (const int*)c->vwid.p,
+                               (const int*)c->icol.p, (long long*)c->vcnt.p);
         } else {
             switch (b + kHashMinLog2) {
                 case 6: launch_sym_hash<6>(c, list, cnt); break;
@@ -886,16 +951,12 @@ int launch_rowptr(spg_context* c) {
 }
 
 // ------------------------------------------------------------------ merge
-// src/mm.cu:366-404 with D1-D4 fixed.  Each virtual row is merged by one
-// workgroup: FP64 accumulators in LDS keyed by column (an open-addressing
-// hash, or a dense tile when the row's column range fits one), first touch
-// by occupancy, then the columns are emitted in ascending order directly at
-// their final CSR position (the symbolic pass fixed every row's offset).
-constexpr int kDenseMinCount = 128;  // denser rows of a narrow range use a tile
-
+// src/mm.cu:366-404 with D1-D4 fixed.  FP64 accumulators in LDS keyed by
+// column, first touch by occupancy (bitmap or hash key, never by value), and
+// columns emitted in ascending order directly at their final CSR position
+// (the symbolic pass fixed every row's offset).
 __global__ void k_vclass_merge(long long V, const long long* __restrict__ vcnt,
-                               const int* __restrict__ vrow_row,
-                               const int* __restrict__ vstart, int n, int tile_log2,
+                               const int* __restrict__ vwid, int W, int n,
                                int* __restrict__ cls) {
     long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
@@ -904,10 +965,13 @@ __global__ void k_vclass_merge(long long V, const long long* __restrict__ vcnt,
         cls[v] = -1;
         return;
     }
-    int lo, width;
-    vrow_range((int)v, vrow_row, vstart, n, tile_log2, lo, width);
-    if (width <= (1 << tile_log2) && (cnt > kDenseMinCount || cnt > kHashMaxCount)) {
-        cls[v] = kClassDense;
+    const int width = vwid[v];
+    if (width <= W) {
+        cls[v] = kClassTile;
+        return;
+    }
+    if (cnt > kRankMinCount && n <= kRankMaxWords * 32 && cnt <= kHashMaxCount) {
+        cls[v] = kClassRank;
         return;
     }
     int lg = kHashMinLog2;
@@ -983,53 +1047,163 @@ __global__ __launch_bounds__(kBlock) void k_merge_hash(
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_merge_dense(
-    const int* __restrict__ vlist, const long long* __restrict__ vbase,
-    const int* __restrict__ icol, const float* __restrict__ ival,
-    const int* __restrict__ vrow_row, const int* __restrict__ vstart, int n, int tile_log2,
+// Tile-width virtual rows: dense FP64 accumulator over the tile's columns
+// plus an occupancy bitmap.  Persistent workgroups; only the touched
+// accumulators are cleared between rows (while they are emitted).
+__global__ __launch_bounds__(kPT) void k_merge_tile(
+    const int* __restrict__ vlist, long long nlist, unsigned long long* __restrict__ ctr,
+    const long long* __restrict__ vbase, const int* __restrict__ vlo,
+    const int* __restrict__ vwid, const int* __restrict__ icol, const float* __restrict__ ival,
     const long long* __restrict__ vout, int* __restrict__ ccol, float* __restrict__ cval) {
     constexpr int WMAX = 1 << kTileMaxLog2;
     constexpr int NW = WMAX / 32;
+    static_assert(NW <= kPT, "one bitmap word per thread");
     __shared__ double acc[WMAX];
     __shared__ unsigned bits[NW];
-    __shared__ int sh[kBlock / 64];
-    const int v = vlist[blockIdx.x];
-    int lo, width;
-    vrow_range(v, vrow_row, vstart, n, tile_log2, lo, width);
-    const int nw = (width + 31) >> 5;
-    for (int w = threadIdx.x; w < nw; w += kBlock) bits[w] = 0u;
-    for (int x = threadIdx.x; x < width; x += kBlock) acc[x] = 0.0;
-    __syncthreads();
-    const long long s1 = vbase[v + 1];
-    for (long long s = vbase[v] + threadIdx.x; s < s1; s += kBlock) {
-        const int cc = icol[s] - lo;
-        atomicOr(&bits[cc >> 5], 1u << (cc & 31));
-        unsafeAtomicAdd(&acc[cc], (double)ival[s]);
-    }
-    __syncthreads();
-    // NW / kBlock words per thread, contiguous, so the emit order is ascending
-    constexpr int WPT = NW / kBlock;
-    const int w0 = threadIdx.x * WPT;
-    int mine = 0;
+    __shared__ int sh[kPT / 64];
+    __shared__ long long sj;
+    for (int x = threadIdx.x; x < WMAX; x += kPT) acc[x] = 0.0;
+    for (int w = threadIdx.x; w < NW; w += kPT) bits[w] = 0u;
+    for (;;) {
+        const long long j = next_item(ctr, &sj);
+        if (j >= nlist) break;
+        const int v = vlist[j];
+        const long long s0 = vbase[v], s1 = vbase[v + 1];
+        const int lo = vlo[v];
+        const int nw = (vwid[v] + 31) >> 5;
+        for (long long s = s0 + threadIdx.x; s < s1; s += 4 * kPT) {
+            int c[4];
+            float x[4];
 #pragma unroll
-    for (int q = 0; q < WPT; ++q) mine += (w0 + q < nw) ? __popc(bits[w0 + q]) : 0;
-    int tot;
-    int pre = block_exscan<int>(mine, &tot, sh);
-    long long o = vout[v] + pre;
-    const long long lim = vout[v + 1];  // slot range fixed by the symbolic pass
+            for (int u = 0; u < 4; ++u) {
+                const long long q = s + (long long)u * kPT;
+                c[u] = -1;
+                x[u] = 0.f;
+                if (q < s1) {
+                    c[u] = icol[q] - lo;
+                    x[u] = ival[q];
+                }
+            }
 #pragma unroll
-    for (int q = 0; q < WPT; ++q) {
-        const int w = w0 + q;
-        if (w >= nw) break;
-        unsigned b = bits[w];
-        while (b && o < lim) {
+            for (int u = 0; u < 4; ++u) {
+                if (c[u] >= 0) {
+                    atomicOr(&bits[c[u] >> 5], 1u << (c[u] & 31));
+                    unsafeAtomicAdd(&acc[c[u]], (double)x[u]);
+                }
+            }
+        }
+        __syncthreads();
+        unsigned b = threadIdx.x < nw ? bits[threadIdx.x] : 0u;
+        int tot;
+        const int pre = block_exscan_n<int, kPT>(__popc(b), &tot, sh);
+        long long o = vout[v] + pre;
+        const long long lim = vout[v + 1];  // slot range fixed by the symbolic pass
+        while (b) {
             const int bit = __ffs(b) - 1;
             b &= b - 1;
-            const int cc = (w << 5) + bit;
-            ccol[o] = lo + cc;
-            cval[o] = (float)acc[cc];
+            const int cc = (threadIdx.x << 5) + bit;
+            const float val = (float)acc[cc];
+            acc[cc] = 0.0;
+            if (o < lim) {
+                ccol[o] = lo + cc;
+                cval[o] = val;
+            }
             ++o;
         }
+        if (threadIdx.x < nw) bits[threadIdx.x] = 0u;
+    }
+}
+
+// Rows over the full column range (n <= 2^18) with 128 < count <= 4096:
+// pass 1 sets the row's column bitmap, a prefix popcount per word gives
+// every column its rank, pass 2 accumulates into acc[rank] in FP64, and the
+// columns are emitted from the bitmap in ascending order (no sort).
+__global__ __launch_bounds__(kPT) void k_merge_rank(
+    const int* __restrict__ vlist, long long nlist, unsigned long long* __restrict__ ctr,
+    const long long* __restrict__ vbase, int n, const int* __restrict__ icol,
+    const float* __restrict__ ival, const long long* __restrict__ vout,
+    int* __restrict__ ccol, float* __restrict__ cval) {
+    constexpr int WPT = kRankMaxWords / kPT;  // 16 words per thread
+    __shared__ unsigned bits[kRankMaxWords];
+    __shared__ unsigned short pre[kRankMaxWords / 2];  // rank before each word pair
+    __shared__ double acc[kHashMaxCount];
+    __shared__ int sh[kPT / 64];
+    __shared__ long long sj;
+    const int nw = (n + 31) >> 5;
+    const int w0 = threadIdx.x * WPT;
+    for (int w = threadIdx.x; w < kRankMaxWords; w += kPT) bits[w] = 0u;
+    for (;;) {
+        const long long j = next_item(ctr, &sj);
+        if (j >= nlist) break;
+        const int v = vlist[j];
+        const long long s0 = vbase[v], s1 = vbase[v + 1];
+        const long long o = vout[v];
+        const int cnt = (int)min((long long)kHashMaxCount, vout[v + 1] - o);
+        for (long long s = s0 + threadIdx.x; s < s1; s += 4 * kPT) {
+            int c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const long long q = s + (long long)u * kPT;
+                c[u] = q < s1 ? icol[q] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (c[u] >= 0) atomicOr(&bits[c[u] >> 5], 1u << (c[u] & 31));
+        }
+        for (int q = threadIdx.x; q < cnt; q += kPT) acc[q] = 0.0;
+        __syncthreads();
+        int mine = 0;
+#pragma unroll
+        for (int q = 0; q < WPT; ++q) mine += (w0 + q < nw) ? __popc(bits[w0 + q]) : 0;
+        int tot;
+        int run = block_exscan_n<int, kPT>(mine, &tot, sh);
+#pragma unroll
+        for (int q = 0; q < WPT; q += 2) {
+            pre[(w0 + q) >> 1] = (unsigned short)run;
+            run += __popc(bits[w0 + q]) + __popc(bits[w0 + q + 1]);
+        }
+        __syncthreads();
+        for (long long s = s0 + threadIdx.x; s < s1; s += 4 * kPT) {
+            int c[4];
+            float x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const long long q = s + (long long)u * kPT;
+                c[u] = -1;
+                x[u] = 0.f;
+                if (q < s1) {
+                    c[u] = icol[q];
+                    x[u] = ival[q];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (c[u] >= 0) {
+                    const int w = c[u] >> 5;
+                    const int r = pre[w >> 1] + ((w & 1) ? __popc(bits[w - 1]) : 0) +
+                                  __popc(bits[w] & ((1u << (c[u] & 31)) - 1u));
+                    if (r < cnt) unsafeAtomicAdd(&acc[r], (double)x[u]);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < WPT; ++q) {
+            const int w = w0 + q;
+            unsigned b = bits[w];
+            int r = pre[w >> 1] + ((w & 1) ? __popc(bits[w - 1]) : 0);
+            while (b) {
+                const int bit = __ffs(b) - 1;
+                b &= b - 1;
+                if (r < cnt) {
+                    ccol[o + r] = (w << 5) + bit;
+                    cval[o + r] = (float)acc[r];
+                }
+                ++r;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < WPT; ++q) bits[w0 + q] = 0u;
     }
 }
 
@@ -1043,35 +1217,35 @@ static void launch_merge_hash(spg_context* c, const int* list, long long cnt) {
 
 int launch_merge(spg_context* c) {
     const long long V = c->V;
+    SPG_TRY(fill<unsigned long long>(c, (unsigned long long*)c->counters.p, 0ull, 64));
     if (V > 0) {
         hipLaunchKernelGGL(k_vclass_merge, dim3((unsigned)((V + kBlock - 1) / kBlock)),
                            dim3(kBlock), 0, c->stream, V, (const long long*)c->vcnt.p,
-                           (const int*)c->vrow_row.p, (const int*)c->vstart.p, c->n,
-                           c->cfg.tile_log2, (int*)c->vclass.p);
+                           (const int*)c->vwid.p, c->W, c->n, (int*)c->vclass.p);
         SPG_HIP(hipGetLastError());
     }
     SPG_TRY(stable_partition(c, (const int32_t*)c->vclass.p, V, kNumVClasses,
                              (int32_t*)c->vlist.p, (int64_t*)c->vclass_off.p));
-    long long* off_h = (long long*)c->pinned_small.p;
-    SPG_HIP(hipMemcpyAsync(off_h, c->vclass_off.p, sizeof(long long) * (kNumVClasses + 1),
-                           hipMemcpyDeviceToHost, c->stream));
-    SPG_HIP(hipStreamSynchronize(c->stream));
     long long off[kNumVClasses + 1];
-    for (int b = 0; b <= kNumVClasses; ++b) off[b] = off_h[b];
+    SPG_TRY(class_offsets(c, off));
     const int* vl = (const int*)c->vlist.p;
-    // dense tiles first (largest), then hash classes from large to small
+    unsigned long long* ctr = (unsigned long long*)c->counters.p;
     tmark(c, SPG_PHASE_MERGE, 0);
     for (int b = kNumVClasses - 1; b >= 0; --b) {
         const long long cnt = off[b + 1] - off[b];
         if (cnt <= 0) continue;
         const int* list = vl + off[b];
-        if (b == kClassDense) {
-            hipLaunchKernelGGL(k_merge_dense, dim3((unsigned)cnt), dim3(kBlock), 0, c->stream,
-                               list, (const long long*)c->vbase.p, (const int*)c->icol.p,
-                               (const float*)c->ival.p, (const int*)c->vrow_row.p,
-                               (const int*)c->vstart.p, c->n, c->cfg.tile_log2,
-                               (const long long*)c->vout.p, (int*)c->ccol.p,
-                               (float*)c->cval.p);
+        if (b == kClassTile) {
+            hipLaunchKernelGGL(k_merge_tile, dim3(persistent_grid(c, cnt, 2)), dim3(kPT), 0,
+                               c->stream, list, cnt, ctr + 1, (const long long*)c->vbase.p,
+                               (const int*)c->vlo.p, (const int*)c->vwid.p,
+                               (const int*)c->icol.p, (const float*)c->ival.p,
+                               (const long long*)c->vout.p, (int*)c->ccol.p, (float*)c->cval.p);
+        } else if (b == kClassRank) {
+            hipLaunchKernelGGL(k_merge_rank, dim3(persistent_grid(c, cnt, 2)), dim3(kPT), 0,
+                               c->stream, list, cnt, ctr + 2, (const long long*)c->vbase.p, c->n,
+                               (const int*)c->icol.p, (const float*)c->ival.p,
+                               (const long long*)c->vout.p, (int*)c->ccol.p, (float*)c->cval.p);
         } else {
             switch (b + kHashMinLog2) {
                 case 6: launch_merge_hash<6>(c, list, cnt); break;
@@ -1109,5 +1283,6 @@ int launch_regroup_intermediate(spg_context* c, int64_t* d_rowptr) {
 
 template int fill<int>(spg_context*, int*, int, long long);
 template int fill<long long>(spg_context*, long long*, long long, long long);
+template int fill<unsigned long long>(spg_context*, unsigned long long*, unsigned long long, long long);
 
 }  // namespace spg

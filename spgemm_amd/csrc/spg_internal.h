@@ -19,8 +19,7 @@ constexpr int kTileMaxLog2 = 13;        // dense tile: 8192 FP64 accumulators
 constexpr int kHashMinLog2 = 6;         // smallest hash: 64 slots
 // Merge/symbolic classes of a virtual row.
 constexpr int kNumHashClasses = kHashMaxLog2 - kHashMinLog2 + 1;  // 64..8192
-constexpr int kClassDense = kNumHashClasses;                      // dense tile
-constexpr int kNumVClasses = kNumHashClasses + 1;
+constexpr int kNumVClasses = kNumHashClasses + 2;  // + tile, rank
 
 // Grow-only device buffer.
 struct DevBuf {
@@ -67,7 +66,9 @@ struct spg_context {
     int stage = 0;  // 0 none, 1 init, 2 categorize, 3 expanded, 4 symbolic, 5 merged
 
     // init products
-    spg::DevBuf ub, ntile, vstart, vrow_row, bsplit, vub, vbase, cursor;
+    spg::DevBuf ub, ntile, vstart, vrow_row, vlo, vwid, bsplit, vub, vbase, cursor;
+    spg::DevBuf counters;  // work counters of persistent kernels
+    int num_cu = 256;
     // categorize products
     spg::DevBuf kbin, kperm, kbin_off, kitems, kitem_off;
     int64_t kbin_off_h[SPG_NUM_KBINS + 1] = {};

@@ -271,3 +271,20 @@ def test_phase_errors(spg, gpu_ctx):
             ctx.set_config(tile_log2=20)
     finally:
         ctx.close()
+
+
+def test_wide_b_hash_path(spg, orc, gpu_ctx):
+    """n > 2^18 columns: rows over the full column range with more than 128
+    distinct columns take the LDS hash + sort path instead of the bitmap
+    rank path."""
+    print("case wide B", flush=True)
+    rng = np.random.default_rng(5)
+    m, K, n = 1500, 1200, 300000
+    ra = np.repeat(np.arange(m), 24).astype(np.int32)
+    ca = rng.integers(0, K, len(ra)).astype(np.int32)
+    rb = np.repeat(np.arange(K), 110).astype(np.int32)
+    cb = rng.integers(0, n, len(rb)).astype(np.int32)
+    A = (ra, ca, rng.uniform(-1, 1, len(ra)).astype(np.float32))
+    B = (rb, cb, rng.uniform(-1, 1, len(rb)).astype(np.float32))
+    compare(run_gpu(spg, gpu_ctx, A, (m, K), B, (K, n), via_coo=False),
+            oracle_all(orc, A, (m, K), B, (K, n)))
