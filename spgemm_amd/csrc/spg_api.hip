@@ -177,7 +177,7 @@ int spgDestroy(spg_context* c) {
     operand_release(c->A);
     operand_release(c->B);
     DevBuf* bufs[] = {&c->ub, &c->ntile, &c->vstart, &c->vrow_row, &c->vlo, &c->vwid,
-                      &c->counters, &c->bsplit, &c->vub,
+                      &c->counters, &c->arow_ptr, &c->arow_k, &c->bsplit, &c->vub,
                       &c->vbase, &c->cursor, &c->kbin, &c->kperm, &c->kbin_off, &c->kitems,
                       &c->kitem_off, &c->icol, &c->ival, &c->vclass, &c->vlist,
                       &c->vclass_off, &c->vcnt, &c->vout, &c->crow, &c->ccol, &c->cval,
@@ -373,6 +373,8 @@ int spgInitGEMM(spg_context* c, cm* C, int64_t* nprod) {
     SPG_TRY(ensure(c, c->ub, sizeof(int64_t) * ((size_t)c->m + 1)));
     SPG_TRY(ensure(c, c->ntile, sizeof(int32_t) * ((size_t)c->m + 1)));
     SPG_TRY(ensure(c, c->vstart, sizeof(int32_t) * ((size_t)c->m + 1)));
+    SPG_TRY(ensure(c, c->arow_ptr, sizeof(int32_t) * ((size_t)c->m + 1)));
+    SPG_TRY(ensure(c, c->arow_k, sizeof(int32_t) * ((size_t)c->A.nnz + 1)));
     SPG_TRY(launch_init(c));
     int32_t V = 0;
     SPG_TRY(read_scalar<int32_t>(c, (const int32_t*)c->vstart.p + c->m, &V));
@@ -388,10 +390,6 @@ int spgInitGEMM(spg_context* c, cm* C, int64_t* nprod) {
     if (c->have_long)
         SPG_TRY(ensure(c, c->bsplit, sizeof(int32_t) * (size_t)c->K * (size_t)(c->T + 1)));
     SPG_TRY(scratch_prepare(c, std::max<long long>(c->V, c->K) + 1));
-    if (c->have_long) {
-        // vub of long rows is accumulated with atomics
-        SPG_TRY(spg::fill<long long>(c, (long long*)c->vub.p, 0, c->V));
-    }
     SPG_TRY(launch_init_vrows(c));
     int64_t np = 0;
     SPG_TRY(read_scalar<int64_t>(c, (const int64_t*)c->vbase.p + c->V, &np));

@@ -68,6 +68,12 @@ __global__ void k_fill(T* p, T v, long long n) {
 }
 
 template <typename T>
+__global__ void k_copy_n(T* __restrict__ dst, const T* __restrict__ src, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+template <typename T>
 int fill(spg_context* c, T* p, T v, long long n) {
     if (n <= 0) return SPG_OK;
     long long nb = (n + kBlock - 1) / kBlock;
@@ -296,26 +302,57 @@ int launch_coo2cm(spg_context* c, const triplet* d_trip, int nnz, int type, int 
 }
 
 // ------------------------------------------------------------------ init
-// src/mm.cu:138-148: every A(i,k) adds nnz(B row k) to the upper bound of
-// C row i.  One wave per column k; the B row length is read once.
-__global__ __launch_bounds__(kBlock) void k_upper_bound(const int* __restrict__ a_ptr,
-                                                         const int* __restrict__ a_idx,
-                                                         const int* __restrict__ b_ptr,
-                                                         int K,
-                                                         unsigned long long* __restrict__ ub) {
+// src/mm.cu:138-148 accumulates the upper bound of C row i with one atomic
+// per A(i,k).  Here A is first transposed into row lists (the k of every
+// entry of row i), and every row then sums its bounds without atomics: hub
+// rows of a power-law A would otherwise serialize thousands of atomics on
+// one counter (and T times that for the per-tile bounds of long rows).
+__global__ __launch_bounds__(kBlock) void k_row_hist(const int* __restrict__ a_ptr,
+                                                      const int* __restrict__ a_idx, int K,
+                                                      int* __restrict__ rcnt) {
     const long long k = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
     if (k >= K) return;
-    const int blen = b_ptr[k + 1] - b_ptr[k];
-    if (blen == 0) return;
     const int e1 = a_ptr[k + 1];
-    for (int e = a_ptr[k] + lane_id(); e < e1; e += 64)
-        atomicAdd(&ub[a_idx[e]], (unsigned long long)blen);
+    for (int e = a_ptr[k] + lane_id(); e < e1; e += 64) atomicAdd(&rcnt[a_idx[e]], 1);
 }
 
-__global__ void k_classify_rows(const long long* __restrict__ ub, int m, long long long_ub,
-                                int T, int* __restrict__ ntile) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) ntile[i] = (T > 1 && ub[i] > long_ub) ? T : 1;
+__global__ __launch_bounds__(kBlock) void k_row_fill(const int* __restrict__ a_ptr,
+                                                      const int* __restrict__ a_idx, int K,
+                                                      int* __restrict__ rcur,
+                                                      int* __restrict__ rk) {
+    const long long k = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    if (k >= K) return;
+    const int e1 = a_ptr[k + 1];
+    for (int e = a_ptr[k] + lane_id(); e < e1; e += 64) rk[atomicAdd(&rcur[a_idx[e]], 1)] = (int)k;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+// One wave per row: ub_i = sum of nnz(B row k) over the row's k; tiles.
+__global__ __launch_bounds__(kBlock) void k_row_bound(const int* __restrict__ rptr,
+                                                       const int* __restrict__ rk,
+                                                       const int* __restrict__ b_ptr, int m,
+                                                       long long long_ub, int T,
+                                                       long long* __restrict__ ub,
+                                                       int* __restrict__ ntile) {
+    const long long i = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    if (i >= m) return;
+    long long s = 0;
+    const int e1 = rptr[i + 1];
+    for (int e = rptr[i] + lane_id(); e < e1; e += 64) {
+        const int k = rk[e];
+        s += b_ptr[k + 1] - b_ptr[k];
+    }
+    s = wave_sum(s);
+    if (lane_id() == 0) {
+        ub[i] = s;
+        ntile[i] = (T > 1 && s > long_ub) ? T : 1;
+    }
 }
 
 __global__ void k_fill_vrows(const int* __restrict__ vstart, const long long* __restrict__ ub,
@@ -359,43 +396,60 @@ __global__ void k_bsplit(const int* __restrict__ b_ptr, const int* __restrict__ 
     bsplit[q] = lo;
 }
 
-// Products of every long-row tile: sum over A(i,k) of the piece of B row k
-// inside tile t.
-__global__ __launch_bounds__(kBlock) void k_tile_bound(const int* __restrict__ a_ptr,
-                                                        const int* __restrict__ a_idx,
-                                                        const int* __restrict__ bsplit,
-                                                        const int* __restrict__ vstart, int K,
-                                                        int T,
-                                                        unsigned long long* __restrict__ vub) {
-    const long long k = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
-    if (k >= K) return;
-    const int* bs = bsplit + k * (T + 1);
-    if (bs[T] == bs[0]) return;
-    const int e1 = a_ptr[k + 1];
-    for (int e = a_ptr[k] + lane_id(); e < e1; e += 64) {
-        const int i = a_idx[e];
-        const int s = vstart[i];
-        if (vstart[i + 1] - s == 1) continue;
-        for (int t = 0; t < T; ++t) {
-            const int len = bs[t + 1] - bs[t];
-            if (len) atomicAdd(&vub[s + t], (unsigned long long)len);
+// Products of every tile of a long row: one wave per row, lane t sums the
+// tile-t pieces of the row's B rows (coalesced reads of the split points).
+__global__ __launch_bounds__(kBlock) void k_row_tile_bound(const int* __restrict__ rptr,
+                                                            const int* __restrict__ rk,
+                                                            const int* __restrict__ bsplit,
+                                                            const int* __restrict__ vstart,
+                                                            int m, int T,
+                                                            long long* __restrict__ vub) {
+    const long long i = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    if (i >= m) return;
+    const int vs = vstart[i];
+    if (vstart[i + 1] - vs == 1) return;
+    const int e0 = rptr[i], e1 = rptr[i + 1];
+    for (int t0 = 0; t0 < T; t0 += 64) {
+        const int t = t0 + lane_id();
+        long long s = 0;
+        if (t < T) {
+            for (int e = e0; e < e1; ++e) {
+                const int* bs = bsplit + (long long)rk[e] * (T + 1);
+                s += bs[t + 1] - bs[t];
+            }
+            vub[vs + t] = s;
         }
     }
 }
 
 int launch_init(spg_context* c) {
     const int m = c->m, K = c->K;
-    long long* ub = (long long*)c->ub.p;
-    SPG_TRY(fill<long long>(c, ub, 0, m));
+    int* rptr = (int*)c->arow_ptr.p;
+    int* rk = (int*)c->arow_k.p;
+    int* rcnt = (int*)scratch_alloc(c, sizeof(int) * ((size_t)m + 1));
+    if (!rcnt) return fail(c, SPG_ENOMEM, "init scratch");
+    SPG_TRY(fill<int>(c, rcnt, 0, m + 1));
+    const long long nbk = ((long long)K * 64 + kBlock - 1) / kBlock;
     if (K > 0) {
-        long long nb = ((long long)K * 64 + kBlock - 1) / kBlock;
-        hipLaunchKernelGGL(k_upper_bound, dim3((unsigned)nb), dim3(kBlock), 0, c->stream,
-                           c->A.ptr, c->A.idx, c->B.ptr, K, (unsigned long long*)ub);
+        hipLaunchKernelGGL(k_row_hist, dim3((unsigned)nbk), dim3(kBlock), 0, c->stream, c->A.ptr,
+                           c->A.idx, K, rcnt);
+        SPG_HIP(hipGetLastError());
+    }
+    SPG_TRY(scan_i32(c, rcnt, rptr, m));
+    // rcnt becomes the fill cursor (a copy of the row starts)
+    hipLaunchKernelGGL(k_copy_n<int>, dim3((m + kBlock) / kBlock), dim3(kBlock), 0, c->stream,
+                       rcnt, (const int*)rptr, (long long)m + 1);
+    SPG_HIP(hipGetLastError());
+    if (K > 0) {
+        hipLaunchKernelGGL(k_row_fill, dim3((unsigned)nbk), dim3(kBlock), 0, c->stream, c->A.ptr,
+                           c->A.idx, K, rcnt, rk);
         SPG_HIP(hipGetLastError());
     }
     if (m > 0) {
-        hipLaunchKernelGGL(k_classify_rows, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                           c->stream, ub, m, (long long)c->cfg.long_row_ub, c->T,
+        const long long nbm = ((long long)m * 64 + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL(k_row_bound, dim3((unsigned)nbm), dim3(kBlock), 0, c->stream,
+                           (const int*)rptr, (const int*)rk, c->B.ptr, m,
+                           (long long)c->cfg.long_row_ub, c->T, (long long*)c->ub.p,
                            (int*)c->ntile.p);
         SPG_HIP(hipGetLastError());
     }
@@ -417,11 +471,11 @@ int launch_init_vrows(spg_context* c) {
                            0, c->stream, c->B.ptr, c->B.idx, K, c->T, c->cfg.tile_log2,
                            (int*)c->bsplit.p);
         SPG_HIP(hipGetLastError());
-        long long nb = ((long long)K * 64 + kBlock - 1) / kBlock;
-        hipLaunchKernelGGL(k_tile_bound, dim3((unsigned)nb), dim3(kBlock), 0, c->stream,
-                           c->A.ptr, c->A.idx, (const int*)c->bsplit.p,
-                           (const int*)c->vstart.p, K, c->T,
-                           (unsigned long long*)c->vub.p);
+        const long long nbm = ((long long)m * 64 + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL(k_row_tile_bound, dim3((unsigned)nbm), dim3(kBlock), 0, c->stream,
+                           (const int*)c->arow_ptr.p, (const int*)c->arow_k.p,
+                           (const int*)c->bsplit.p, (const int*)c->vstart.p, m, c->T,
+                           (long long*)c->vub.p);
         SPG_HIP(hipGetLastError());
     }
     SPG_TRY(scan_i64(c, (const int64_t*)c->vub.p, (int64_t*)c->vbase.p, c->V));
@@ -508,6 +562,22 @@ int launch_categorize(spg_context* c) {
     return SPG_OK;
 }
 
+// Persistent loop helper: the workgroup takes list entries from a global
+// counter (largest rows come first in the list, so the tail is short).
+__device__ __forceinline__ long long next_item(unsigned long long* ctr, long long* sj) {
+    __syncthreads();
+    if (threadIdx.x == 0) *sj = (long long)atomicAdd(ctr, 1ull);
+    __syncthreads();
+    return *sj;
+}
+
+// Grid of a persistent kernel: every workgroup that can be resident.
+static unsigned persistent_grid(spg_context* c, long long nlist, int per_cu) {
+    long long g = (long long)c->num_cu * per_cu;
+    if (g > nlist) g = nlist;
+    return (unsigned)(g > 0 ? g : 1);
+}
+
 // -------------------------------------------------------------- expansion
 // src/mm.cu:288-312 (binGEMM) / 332-354 (simpleGEMM): for the A entries of
 // one column k and B row k, every product a_ik * b_kj goes to a slot of C
@@ -515,7 +585,8 @@ int launch_categorize(spg_context* c) {
 // the reference reserves per row (src/mm.cu:304); lanes then write runs of
 // consecutive slots.  A group of G lanes (G = next power of two >= the B row
 // piece, at most 64) serves one A entry, so short B rows do not idle lanes.
-constexpr int kTBCap = 2048;  // LDS slot bases of long-row tiles per batch
+constexpr int kTBCap = 2048;     // LDS slot bases of long-row tiles per batch
+constexpr int kItemChunk = 8;    // work items taken per counter increment
 
 struct ExpandArgs {
     const int* a_ptr;
@@ -526,11 +597,9 @@ struct ExpandArgs {
     const float* b_val;
     const int* kperm;
     const long long* kitem_off;
-    long long p_begin, p_end;   // positions in kperm covered by this launch
-    long long item_begin;       // first work item of this launch
-    long long nitems;
+    long long np;               // positions in kperm (binned k)
+    long long nitems;           // work items
     int P;                      // products per work item
-    int simple;                 // 1: one workgroup per k, no item table
     const int* vstart;
     const long long* vbase;
     int* cursor;
@@ -541,118 +610,147 @@ struct ExpandArgs {
     float* ival;
 };
 
-__global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs g) {
-    __shared__ long long sbase[kBlock];
-    __shared__ float sa[kBlock];
-    __shared__ int sflag[kBlock];
-    __shared__ long long tbase[kTBCap];
-    __shared__ int sbs[kTBCap / 2 + 1];
+struct ExpandLds {
+    long long sbase[kBlock];
+    float sa[kBlock];
+    int sflag[kBlock];
+    long long tbase[kTBCap];
+    int sbs[kTBCap / 2 + 1];
+};
 
-    long long bid = blockIdx.x;
-    if (g.swizzle) bid = xcd_swizzle(bid, gridDim.x);
-    int k;
-    long long e_begin, ne;
-    int j0, nj;
-    int a0, b0;
-    if (g.simple) {
-        k = (int)bid;
-        a0 = g.a_ptr[k];
-        b0 = g.b_ptr[k];
-        e_begin = a0;
-        ne = g.a_ptr[k + 1] - a0;
-        j0 = 0;
-        nj = g.b_ptr[k + 1] - b0;
-        if (ne == 0 || nj == 0) return;
-    } else {
-        const long long item = g.item_begin + bid;
-        long long lo = g.p_begin, hi = g.p_end;  // largest p with kitem_off[p] <= item
-        while (hi - lo > 1) {
-            long long mid = (lo + hi) >> 1;
-            if (g.kitem_off[mid] <= item) lo = mid;
-            else hi = mid;
-        }
-        k = g.kperm[lo];
-        const long long local = item - g.kitem_off[lo];
-        a0 = g.a_ptr[k];
-        b0 = g.b_ptr[k];
-        const long long alen = g.a_ptr[k + 1] - a0;
-        const int blen = g.b_ptr[k + 1] - b0;
-        if (blen <= g.P) {
-            long long epi = g.P / blen;
-            if (epi < 1) epi = 1;
-            e_begin = a0 + local * epi;
-            ne = alen - local * epi;
-            if (ne > epi) ne = epi;
-            j0 = 0;
-            nj = blen;
-        } else {
-            const long long jch = (blen + g.P - 1) / g.P;
-            e_begin = a0 + local / jch;
-            ne = 1;
-            j0 = (int)(local % jch) * g.P;
-            nj = blen - j0;
-            if (nj > g.P) nj = g.P;
-        }
-    }
-    const int lg = g.simple ? 6 : min(6, 32 - __clz(max(nj, 1) - 1));  // ceil(log2 nj)
+// Expands A entries [e_begin, e_begin+ne) of column k against B row k's
+// positions [j0, j0+nj).  G lanes per entry; every thread of the workgroup
+// calls it with the same arguments.
+__device__ void expand_work(const ExpandArgs& g, ExpandLds& L, int k, long long e_begin,
+                            long long ne, int j0, int nj, int lg) {
     const int G = 1 << lg;
     int NE = kBlock >> lg;
     const int T = g.T;
+    const int b0 = g.b_ptr[k];
+    __syncthreads();  // LDS of the previous item is free
     if (T > 1) {
-        int cap = kTBCap / T;
+        const int cap = kTBCap / T;
         if (cap < NE) NE = cap;
-        for (int t = threadIdx.x; t <= T; t += kBlock) sbs[t] = g.bsplit[(long long)k * (T + 1) + t];
+        for (int t = threadIdx.x; t <= T; t += kBlock) L.sbs[t] = g.bsplit[(long long)k * (T + 1) + t];
     }
     const int jlo = b0 + j0, jhi = b0 + j0 + nj;
-
     for (long long eb = 0; eb < ne; eb += NE) {
         const int cnt = (int)min((long long)NE, ne - eb);
-        __syncthreads();
+        if (eb > 0) __syncthreads();
         if (threadIdx.x < cnt) {
             const long long e = e_begin + eb + threadIdx.x;
             const int i = g.a_idx[e];
-            sa[threadIdx.x] = g.a_val[e];
+            L.sa[threadIdx.x] = g.a_val[e];
             const int vs = g.vstart[i];
             if (g.vstart[i + 1] - vs == 1) {
                 const int off = atomicAdd(&g.cursor[vs], nj);
-                sbase[threadIdx.x] = g.vbase[vs] + off;
-                sflag[threadIdx.x] = 0;
+                L.sbase[threadIdx.x] = g.vbase[vs] + off;
+                L.sflag[threadIdx.x] = 0;
             } else {
-                sbase[threadIdx.x] = vs;
-                sflag[threadIdx.x] = 1;
+                L.sbase[threadIdx.x] = vs;
+                L.sflag[threadIdx.x] = 1;
             }
         }
         __syncthreads();
         if (T > 1) {
             for (int pr = threadIdx.x; pr < cnt * T; pr += kBlock) {
                 const int q = pr / T, t = pr - q * T;
-                if (!sflag[q]) continue;
-                const int s = max(sbs[t], jlo), e2 = min(sbs[t + 1], jhi);
+                if (!L.sflag[q]) continue;
+                const int s = max(L.sbs[t], jlo), e2 = min(L.sbs[t + 1], jhi);
                 if (e2 > s) {
-                    const int v = (int)sbase[q] + t;
+                    const int v = (int)L.sbase[q] + t;
                     const int off = atomicAdd(&g.cursor[v], e2 - s);
-                    tbase[q * T + t] = g.vbase[v] + off - s;
+                    L.tbase[q * T + t] = g.vbase[v] + off - s;
                 }
             }
             __syncthreads();
         }
         const int q = threadIdx.x >> lg;
         if (q < cnt) {
-            const float a = sa[q];
-            const int fl = sflag[q];
-            const long long base = sbase[q];
+            const float a = L.sa[q];
+            const int fl = L.sflag[q];
+            const long long base = L.sbase[q];
             for (int jj = threadIdx.x & (G - 1); jj < nj; jj += G) {
                 const int j = jlo + jj;
                 const int col = g.b_idx[j];
                 const float p = __fmul_rn(a, g.b_val[j]);
                 long long slot;
                 if (!fl) slot = base + jj;
-                else slot = tbase[q * T + (col >> g.tile_log2)] + j;
+                else slot = L.tbase[q * T + (col >> g.tile_log2)] + j;
                 g.icol[slot] = col;
                 g.ival[slot] = p;
             }
         }
     }
+}
+
+__device__ __forceinline__ int ceil_log2_6(int nj) {
+    return min(6, 32 - __clz(max(nj, 1) - 1));
+}
+
+// Binned expansion: persistent workgroups take chunks of kItemChunk
+// consecutive work items (the k-bins are contiguous in item order, heavy
+// bin first).  With the XCD mapping on, chunk c is served by the XCD c % 8
+// so the items of one chunk (mostly one k) share that XCD's L2 copy of B
+// row k.
+__global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs g,
+                                                    unsigned long long* __restrict__ ctr) {
+    __shared__ ExpandLds L;
+    __shared__ long long sj;
+    const int parts = g.swizzle ? 8 : 1;
+    const int part = g.swizzle ? (int)(blockIdx.x & 7) : 0;
+    const long long nchunks = (g.nitems + kItemChunk - 1) / kItemChunk;
+    for (;;) {
+        const long long r = next_item(ctr + part, &sj);
+        const long long chunk = r * parts + part;
+        if (chunk >= nchunks) break;
+        const long long first = chunk * kItemChunk;
+        const long long last = min(first + kItemChunk, g.nitems);
+        long long lo = 0, hi = g.np;  // largest p with kitem_off[p] <= first
+        while (hi - lo > 1) {
+            const long long mid = (lo + hi) >> 1;
+            if (g.kitem_off[mid] <= first) lo = mid;
+            else hi = mid;
+        }
+        long long p = lo;
+        for (long long item = first; item < last; ++item) {
+            while (g.kitem_off[p + 1] <= item) ++p;
+            const int k = g.kperm[p];
+            const long long local = item - g.kitem_off[p];
+            const int a0 = g.a_ptr[k];
+            const int b0 = g.b_ptr[k];
+            const long long alen = g.a_ptr[k + 1] - a0;
+            const int blen = g.b_ptr[k + 1] - b0;
+            long long e_begin, ne;
+            int j0, nj;
+            if (blen <= g.P) {
+                long long epi = g.P / blen;
+                if (epi < 1) epi = 1;
+                e_begin = a0 + local * epi;
+                ne = min(epi, alen - local * epi);
+                j0 = 0;
+                nj = blen;
+            } else {
+                const long long jch = (blen + g.P - 1) / g.P;
+                e_begin = a0 + local / jch;
+                ne = 1;
+                j0 = (int)(local % jch) * g.P;
+                nj = min(g.P, blen - j0);
+            }
+            expand_work(g, L, k, e_begin, ne, j0, nj, ceil_log2_6(nj));
+        }
+    }
+}
+
+// Plain expansion (simpleGEMM): one workgroup per k, 64 lanes per entry.
+__global__ __launch_bounds__(kBlock) void k_expand_simple(ExpandArgs g) {
+    __shared__ ExpandLds L;
+    const int k = blockIdx.x;
+    const int a0 = g.a_ptr[k], b0 = g.b_ptr[k];
+    const long long ne = g.a_ptr[k + 1] - a0;
+    const int nj = g.b_ptr[k + 1] - b0;
+    if (ne == 0 || nj == 0) return;
+    expand_work(g, L, k, a0, ne, 0, nj, 6);
 }
 
 static ExpandArgs expand_args(spg_context* c) {
@@ -680,30 +778,23 @@ static ExpandArgs expand_args(spg_context* c) {
 
 int launch_expand_binned(spg_context* c) {
     ExpandArgs g = expand_args(c);
-    // heavy bin last: its items are the largest, the others fill the GPU
-    static const int order[SPG_NUM_KBINS] = {1, 2, 3, 4, 5, 6, 7, 8, 0};
-    for (int oi = 0; oi < SPG_NUM_KBINS; ++oi) {
-        const int b = order[oi];
-        const long long i0 = c->kitems_bin_h[b], i1 = c->kitems_bin_h[b + 1];
-        if (i1 <= i0) continue;
-        g.p_begin = c->kbin_off_h[b];
-        g.p_end = c->kbin_off_h[b + 1];
-        g.item_begin = i0;
-        g.nitems = i1 - i0;
-        g.simple = 0;
-        if (g.nitems > 0x7fffffffLL) return fail(c, SPG_EOVERFLOW, "too many work items");
-        hipLaunchKernelGGL(k_expand, dim3((unsigned)g.nitems), dim3(kBlock), 0, c->stream, g);
-        SPG_HIP(hipGetLastError());
-    }
+    g.np = c->kbin_off_h[SPG_NUM_KBINS];
+    g.nitems = c->kitems_bin_h[SPG_NUM_KBINS];
+    if (g.nitems <= 0) return SPG_OK;
+    unsigned long long* ctr = (unsigned long long*)c->counters.p + 8;
+    SPG_TRY(fill<unsigned long long>(c, ctr, 0ull, 8));
+    const long long nchunks = (g.nitems + kItemChunk - 1) / kItemChunk;
+    unsigned grid = persistent_grid(c, nchunks, 6);
+    if (g.swizzle) grid = (grid + 7) & ~7u;  // every XCD part gets workgroups
+    hipLaunchKernelGGL(k_expand, dim3(grid), dim3(kBlock), 0, c->stream, g, ctr);
+    SPG_HIP(hipGetLastError());
     return SPG_OK;
 }
 
 int launch_expand_simple(spg_context* c) {
     ExpandArgs g = expand_args(c);
-    g.simple = 1;
-    g.swizzle = 0;
     if (c->K > 0) {
-        hipLaunchKernelGGL(k_expand, dim3((unsigned)c->K), dim3(kBlock), 0, c->stream, g);
+        hipLaunchKernelGGL(k_expand_simple, dim3((unsigned)c->K), dim3(kBlock), 0, c->stream, g);
         SPG_HIP(hipGetLastError());
     }
     return SPG_OK;
@@ -817,53 +908,52 @@ __device__ __forceinline__ T block_exscan_n(T x, T* total, T* sh) {
     return wpre + inc - x;
 }
 
-// Persistent loop helper: the workgroup takes list entries from a global
-// counter (largest rows come first in the list, so the tail is short).
-__device__ __forceinline__ long long next_item(unsigned long long* ctr, long long* sj) {
-    __syncthreads();
-    if (threadIdx.x == 0) *sj = (long long)atomicAdd(ctr, 1ull);
-    __syncthreads();
-    return *sj;
-}
+// Symbolic count of tile-width virtual rows: one wave per row with a
+// wave-private bitmap of the tile's columns (no workgroup barriers, so the
+// 32 waves of a CU overlap their load latencies); four product loads are in
+// flight per lane before the LDS updates.
+constexpr int kSymWaves = kPT / 64;
 
-// Symbolic count of tile-width virtual rows: a bitmap of the tile's columns.
-// Four product loads are in flight per thread before the LDS updates.
 __global__ __launch_bounds__(kPT) void k_symbolic_tile(
     const int* __restrict__ vlist, long long nlist, unsigned long long* __restrict__ ctr,
     const long long* __restrict__ vbase, const int* __restrict__ vlo,
     const int* __restrict__ vwid, const int* __restrict__ icol, long long* __restrict__ vcnt) {
     constexpr int NW = (1 << kTileMaxLog2) / 32;
-    __shared__ unsigned bits[NW];
-    __shared__ int sh[kPT / 64];
-    __shared__ long long sj;
-    for (int w = threadIdx.x; w < NW; w += kPT) bits[w] = 0u;
+    __shared__ unsigned bitsall[kSymWaves][NW];
+    const int wv = threadIdx.x >> 6, lane = lane_id();
+    unsigned* bits = bitsall[wv];
+    for (int w = lane; w < NW; w += 64) bits[w] = 0u;
     for (;;) {
-        const long long j = next_item(ctr, &sj);
+        long long j = 0;
+        if (lane == 0) j = (long long)atomicAdd(ctr, 1ull);
+        j = __shfl(j, 0, 64);
         if (j >= nlist) break;
         const int v = vlist[j];
         const long long s0 = vbase[v], s1 = vbase[v + 1];
         const int lo = vlo[v];
         const int nw = (vwid[v] + 31) >> 5;
-        for (long long s = s0 + threadIdx.x; s < s1; s += 4 * kPT) {
+        for (long long s = s0 + lane; s < s1; s += 4 * 64) {
             int c[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const long long q = s + (long long)u * kPT;
+                const long long q = s + u * 64;
                 c[u] = q < s1 ? icol[q] - lo : -1;
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u)
                 if (c[u] >= 0) atomicOr(&bits[c[u] >> 5], 1u << (c[u] & 31));
         }
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
         int mine = 0;
-        for (int w = threadIdx.x; w < nw; w += kPT) {
+        for (int w = lane; w < nw; w += 64) {
             mine += __popc(bits[w]);
             bits[w] = 0u;
         }
-        int tot;
-        block_exscan_n<int, kPT>(mine, &tot, sh);
-        if (threadIdx.x == 0) vcnt[v] = tot;
+        mine = wave_sum(mine);
+        if (lane == 0) vcnt[v] = mine;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -878,13 +968,6 @@ static void launch_sym_hash(spg_context* c, const int* list, long long cnt) {
     hipLaunchKernelGGL(k_symbolic_hash<LOG2H>, dim3((unsigned)cnt), dim3(kBlock), 0, c->stream,
                        list, (const long long*)c->vbase.p, (const int*)c->icol.p,
                        (long long*)c->vcnt.p);
-}
-
-// Grid of a persistent kernel: every workgroup that can be resident.
-static unsigned persistent_grid(spg_context* c, long long nlist, int per_cu) {
-    long long g = (long long)c->num_cu * per_cu;
-    if (g > nlist) g = nlist;
-    return (unsigned)(g > 0 ? g : 1);
 }
 
 static int class_offsets(spg_context* c, long long* off) {
@@ -918,7 +1001,9 @@ int launch_symbolic(spg_context* c) {
         if (cnt <= 0) continue;
         const int* list = vl + off[b];
         if (b == kClassTile) {
-            hipLaunchKernelGGL(k_symbolic_tile, dim3(persistent_grid(c, cnt, 4)), dim3(kPT), 0,
+            hipLaunchKernelGGL(k_symbolic_tile,
+                               dim3(persistent_grid(c, (cnt + kSymWaves - 1) / kSymWaves, 4)),
+                               dim3(kPT), 0,
                                c->stream, list, cnt, ctr + 0, (const long long*)c->vbase.p,
                                (const int*)c->vlo.p, (const int*)c->vwid.p,
                                (const int*)c->icol.p, (long long*)c->vcnt.p);
@@ -1048,69 +1133,106 @@ __global__ __launch_bounds__(kBlock) void k_merge_hash(
 }
 
 // Tile-width virtual rows: dense FP64 accumulator over the tile's columns
-// plus an occupancy bitmap.  Persistent workgroups; only the touched
-// accumulators are cleared between rows (while they are emitted).
+// plus an occupancy bitmap.  Persistent workgroups walk the list with a
+// fixed stride; the next row's descriptor and first 4*kPT products are
+// loaded before the current row is emitted, so their latency overlaps the
+// emission.  Only touched accumulators are cleared (while being emitted).
+__device__ __forceinline__ void load4(const int* __restrict__ icol,
+                                      const float* __restrict__ ival, long long s,
+                                      long long s1, int lo, int (&c)[4], float (&x)[4]) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const long long q = s + (long long)u * kPT;
+        c[u] = -1;
+        x[u] = 0.f;
+        if (q < s1) {
+            c[u] = icol[q] - lo;
+            x[u] = ival[q];
+        }
+    }
+}
+
+__device__ __forceinline__ void apply4(double* acc, unsigned* bits, const int (&c)[4],
+                                       const float (&x)[4]) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        if (c[u] >= 0) {
+            atomicOr(&bits[c[u] >> 5], 1u << (c[u] & 31));
+            unsafeAtomicAdd(&acc[c[u]], (double)x[u]);
+        }
+    }
+}
+
 __global__ __launch_bounds__(kPT) void k_merge_tile(
-    const int* __restrict__ vlist, long long nlist, unsigned long long* __restrict__ ctr,
-    const long long* __restrict__ vbase, const int* __restrict__ vlo,
-    const int* __restrict__ vwid, const int* __restrict__ icol, const float* __restrict__ ival,
-    const long long* __restrict__ vout, int* __restrict__ ccol, float* __restrict__ cval) {
+    const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
+    const int* __restrict__ vlo, const int* __restrict__ vwid, const int* __restrict__ icol,
+    const float* __restrict__ ival, const long long* __restrict__ vout, int* __restrict__ ccol,
+    float* __restrict__ cval) {
     constexpr int WMAX = 1 << kTileMaxLog2;
     constexpr int NW = WMAX / 32;
     static_assert(NW <= kPT, "one bitmap word per thread");
     __shared__ double acc[WMAX];
     __shared__ unsigned bits[NW];
     __shared__ int sh[kPT / 64];
-    __shared__ long long sj;
     for (int x = threadIdx.x; x < WMAX; x += kPT) acc[x] = 0.0;
     for (int w = threadIdx.x; w < NW; w += kPT) bits[w] = 0u;
+    __syncthreads();
+    long long j = blockIdx.x;
+    if (j >= nlist) return;
+    const long long G = gridDim.x;
+    int v = vlist[j];
+    long long s0 = vbase[v], s1 = vbase[v + 1], o0 = vout[v], lim = vout[v + 1];
+    int lo = vlo[v], nw = (vwid[v] + 31) >> 5;
+    int c[4];
+    float x[4];
+    load4(icol, ival, s0 + threadIdx.x, s1, lo, c, x);
     for (;;) {
-        const long long j = next_item(ctr, &sj);
-        if (j >= nlist) break;
-        const int v = vlist[j];
-        const long long s0 = vbase[v], s1 = vbase[v + 1];
-        const int lo = vlo[v];
-        const int nw = (vwid[v] + 31) >> 5;
-        for (long long s = s0 + threadIdx.x; s < s1; s += 4 * kPT) {
-            int c[4];
-            float x[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const long long q = s + (long long)u * kPT;
-                c[u] = -1;
-                x[u] = 0.f;
-                if (q < s1) {
-                    c[u] = icol[q] - lo;
-                    x[u] = ival[q];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (c[u] >= 0) {
-                    atomicOr(&bits[c[u] >> 5], 1u << (c[u] & 31));
-                    unsafeAtomicAdd(&acc[c[u]], (double)x[u]);
-                }
-            }
+        const long long jn = j + G;
+        const int vn = jn < nlist ? vlist[jn] : -1;
+        apply4(acc, bits, c, x);
+        for (long long s = s0 + threadIdx.x + 4 * kPT; s < s1; s += 4 * kPT) {
+            load4(icol, ival, s, s1, lo, c, x);
+            apply4(acc, bits, c, x);
         }
         __syncthreads();
+        long long s0n = 0, s1n = 0, o0n = 0, limn = 0;
+        int lon = 0, nwn = 0;
+        if (vn >= 0) {
+            s0n = vbase[vn];
+            s1n = vbase[vn + 1];
+            o0n = vout[vn];
+            limn = vout[vn + 1];
+            lon = vlo[vn];
+            nwn = (vwid[vn] + 31) >> 5;
+            load4(icol, ival, s0n + threadIdx.x, s1n, lon, c, x);
+        }
         unsigned b = threadIdx.x < nw ? bits[threadIdx.x] : 0u;
         int tot;
         const int pre = block_exscan_n<int, kPT>(__popc(b), &tot, sh);
-        long long o = vout[v] + pre;
-        const long long lim = vout[v + 1];  // slot range fixed by the symbolic pass
+        long long o = o0 + pre;
         while (b) {
             const int bit = __ffs(b) - 1;
             b &= b - 1;
             const int cc = (threadIdx.x << 5) + bit;
             const float val = (float)acc[cc];
             acc[cc] = 0.0;
-            if (o < lim) {
+            if (o < lim) {  // slot range fixed by the symbolic pass
                 ccol[o] = lo + cc;
                 cval[o] = val;
             }
             ++o;
         }
         if (threadIdx.x < nw) bits[threadIdx.x] = 0u;
+        __syncthreads();
+        if (vn < 0) break;
+        j = jn;
+        v = vn;
+        s0 = s0n;
+        s1 = s1n;
+        o0 = o0n;
+        lim = limn;
+        lo = lon;
+        nw = nwn;
     }
 }
 
@@ -1237,7 +1359,7 @@ int launch_merge(spg_context* c) {
         const int* list = vl + off[b];
         if (b == kClassTile) {
             hipLaunchKernelGGL(k_merge_tile, dim3(persistent_grid(c, cnt, 2)), dim3(kPT), 0,
-                               c->stream, list, cnt, ctr + 1, (const long long*)c->vbase.p,
+                               c->stream, list, cnt, (const long long*)c->vbase.p,
                                (const int*)c->vlo.p, (const int*)c->vwid.p,
                                (const int*)c->icol.p, (const float*)c->ival.p,
                                (const long long*)c->vout.p, (int*)c->ccol.p, (float*)c->cval.p);

@@ -68,6 +68,7 @@ struct spg_context {
     // init products
     spg::DevBuf ub, ntile, vstart, vrow_row, vlo, vwid, bsplit, vub, vbase, cursor;
     spg::DevBuf counters;  // work counters of persistent kernels
+    spg::DevBuf arow_ptr, arow_k;  // A transposed: row i -> its column indices k
     int num_cu = 256;
     // categorize products
     spg::DevBuf kbin, kperm, kbin_off, kitems, kitem_off;
