@@ -396,29 +396,55 @@ __global__ void k_bsplit(const int* __restrict__ b_ptr, const int* __restrict__ 
     bsplit[q] = lo;
 }
 
-// Products of every tile of a long row: one wave per row, lane t sums the
-// tile-t pieces of the row's B rows (coalesced reads of the split points).
+// Products of every tile of a long row: one workgroup per row; lane t of
+// every wave sums the tile-t pieces of its share of the row's B rows
+// (coalesced reads of the split points, four rows in flight), then the
+// waves' partial sums are added in LDS.
 __global__ __launch_bounds__(kBlock) void k_row_tile_bound(const int* __restrict__ rptr,
                                                             const int* __restrict__ rk,
                                                             const int* __restrict__ bsplit,
                                                             const int* __restrict__ vstart,
                                                             int m, int T,
                                                             long long* __restrict__ vub) {
-    const long long i = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
-    if (i >= m) return;
+    __shared__ long long part[kBlock / 64][64];
+    const int i = blockIdx.x;
     const int vs = vstart[i];
     if (vstart[i + 1] - vs == 1) return;
+    const int wv = threadIdx.x >> 6, lane = lane_id();
+    constexpr int NWV = kBlock / 64;
     const int e0 = rptr[i], e1 = rptr[i + 1];
     for (int t0 = 0; t0 < T; t0 += 64) {
-        const int t = t0 + lane_id();
+        const int t = t0 + lane;
         long long s = 0;
         if (t < T) {
-            for (int e = e0; e < e1; ++e) {
+            int e = e0 + wv;
+            for (; e + 3 * NWV < e1; e += 4 * NWV) {
+                int kk[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) kk[u] = rk[e + u * NWV];
+                int d[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int* bs = bsplit + (long long)kk[u] * (T + 1);
+                    d[u] = bs[t + 1] - bs[t];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) s += d[u];
+            }
+            for (; e < e1; e += NWV) {
                 const int* bs = bsplit + (long long)rk[e] * (T + 1);
                 s += bs[t + 1] - bs[t];
             }
-            vub[vs + t] = s;
         }
+        part[wv][lane] = s;
+        __syncthreads();
+        if (wv == 0 && t < T) {
+            long long tot = 0;
+#pragma unroll
+            for (int q = 0; q < NWV; ++q) tot += part[q][lane];
+            vub[vs + t] = tot;
+        }
+        __syncthreads();
     }
 }
 
@@ -471,8 +497,7 @@ int launch_init_vrows(spg_context* c) {
                            0, c->stream, c->B.ptr, c->B.idx, K, c->T, c->cfg.tile_log2,
                            (int*)c->bsplit.p);
         SPG_HIP(hipGetLastError());
-        const long long nbm = ((long long)m * 64 + kBlock - 1) / kBlock;
-        hipLaunchKernelGGL(k_row_tile_bound, dim3((unsigned)nbm), dim3(kBlock), 0, c->stream,
+        hipLaunchKernelGGL(k_row_tile_bound, dim3((unsigned)m), dim3(kBlock), 0, c->stream,
                            (const int*)c->arow_ptr.p, (const int*)c->arow_k.p,
                            (const int*)c->bsplit.p, (const int*)c->vstart.p, m, c->T,
                            (long long*)c->vub.p);
@@ -670,15 +695,30 @@ __device__ void expand_work(const ExpandArgs& g, ExpandLds& L, int k, long long 
             const float a = L.sa[q];
             const int fl = L.sflag[q];
             const long long base = L.sbase[q];
-            for (int jj = threadIdx.x & (G - 1); jj < nj; jj += G) {
-                const int j = jlo + jj;
-                const int col = g.b_idx[j];
-                const float p = __fmul_rn(a, g.b_val[j]);
-                long long slot;
-                if (!fl) slot = base + jj;
-                else slot = L.tbase[q * T + (col >> g.tile_log2)] + j;
-                g.icol[slot] = col;
-                g.ival[slot] = p;
+            const int* __restrict__ bidx = g.b_idx;
+            const float* __restrict__ bval = g.b_val;
+            int* __restrict__ icol = g.icol;
+            float* __restrict__ ival = g.ival;
+            for (int jj0 = threadIdx.x & (G - 1); jj0 < nj; jj0 += 4 * G) {
+                int col[4];
+                float bv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int jj = jj0 + u * G;
+                    col[u] = jj < nj ? bidx[jlo + jj] : 0;
+                    bv[u] = jj < nj ? bval[jlo + jj] : 0.f;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int jj = jj0 + u * G;
+                    if (jj < nj) {
+                        long long slot;
+                        if (!fl) slot = base + jj;
+                        else slot = L.tbase[q * T + (col[u] >> g.tile_log2)] + jlo + jj;
+                        icol[slot] = col[u];
+                        ival[slot] = __fmul_rn(a, bv[u]);
+                    }
+                }
             }
         }
     }
@@ -798,6 +838,25 @@ int launch_expand_simple(spg_context* c) {
         SPG_HIP(hipGetLastError());
     }
     return SPG_OK;
+}
+
+// Sets bit c (c < 0: no bit) of an LDS bitmap.  Products of one k arrive in
+// ascending column order, so neighbouring lanes usually hit the same 32-bit
+// word; lanes first OR their bits along runs of equal words (segmented
+// scan over the wave) and only the last lane of a run issues the LDS
+// atomic, instead of up to 64 serialized atomics on one address.
+__device__ __forceinline__ void wave_bitmap_or(unsigned* bits, int c) {
+    const int lane = lane_id();
+    const int w = c >= 0 ? (c >> 5) : -1 - lane;  // unique dummy word for idle lanes
+    unsigned m = c >= 0 ? (1u << (c & 31)) : 0u;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int wp = __shfl_up(w, off, 64);
+        const unsigned mp = __shfl_up(m, off, 64);
+        if (lane >= off && wp == w) m |= mp;
+    }
+    const int wn = __shfl_down(w, 1, 64);
+    if (c >= 0 && (lane == 63 || wn != w)) atomicOr(&bits[w], m);
 }
 
 __device__ __forceinline__ unsigned hash_slot(int key, int log2h) {
@@ -932,16 +991,16 @@ __global__ __launch_bounds__(kPT) void k_symbolic_tile(
         const long long s0 = vbase[v], s1 = vbase[v + 1];
         const int lo = vlo[v];
         const int nw = (vwid[v] + 31) >> 5;
-        for (long long s = s0 + lane; s < s1; s += 4 * 64) {
+        // trip count uniform across the wave (the OR below shuffles)
+        for (long long sb = s0; sb < s1; sb += 4 * 64) {
             int c[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const long long q = s + u * 64;
+                const long long q = sb + lane + u * 64;
                 c[u] = q < s1 ? icol[q] - lo : -1;
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (c[u] >= 0) atomicOr(&bits[c[u] >> 5], 1u << (c[u] & 31));
+            for (int u = 0; u < 4; ++u) wave_bitmap_or(bits, c[u]);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -1156,10 +1215,8 @@ __device__ __forceinline__ void apply4(double* acc, unsigned* bits, const int (&
                                        const float (&x)[4]) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        if (c[u] >= 0) {
-            atomicOr(&bits[c[u] >> 5], 1u << (c[u] & 31));
-            unsafeAtomicAdd(&acc[c[u]], (double)x[u]);
-        }
+        wave_bitmap_or(bits, c[u]);
+        if (c[u] >= 0) unsafeAtomicAdd(&acc[c[u]], (double)x[u]);
     }
 }
 
@@ -1190,8 +1247,9 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
         const long long jn = j + G;
         const int vn = jn < nlist ? vlist[jn] : -1;
         apply4(acc, bits, c, x);
-        for (long long s = s0 + threadIdx.x + 4 * kPT; s < s1; s += 4 * kPT) {
-            load4(icol, ival, s, s1, lo, c, x);
+        // trip count uniform across the workgroup (apply4 shuffles)
+        for (long long sb = s0 + 4 * kPT; sb < s1; sb += 4 * kPT) {
+            load4(icol, ival, sb + threadIdx.x, s1, lo, c, x);
             apply4(acc, bits, c, x);
         }
         __syncthreads();
@@ -1261,16 +1319,16 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
         const long long s0 = vbase[v], s1 = vbase[v + 1];
         const long long o = vout[v];
         const int cnt = (int)min((long long)kHashMaxCount, vout[v + 1] - o);
-        for (long long s = s0 + threadIdx.x; s < s1; s += 4 * kPT) {
+        // trip count uniform across the workgroup (the OR below shuffles)
+        for (long long sb = s0; sb < s1; sb += 4 * kPT) {
             int c[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const long long q = s + (long long)u * kPT;
+                const long long q = sb + threadIdx.x + (long long)u * kPT;
                 c[u] = q < s1 ? icol[q] : -1;
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (c[u] >= 0) atomicOr(&bits[c[u] >> 5], 1u << (c[u] & 31));
+            for (int u = 0; u < 4; ++u) wave_bitmap_or(bits, c[u]);
         }
         for (int q = threadIdx.x; q < cnt; q += kPT) acc[q] = 0.0;
         __syncthreads();
