@@ -177,7 +177,8 @@ int spgDestroy(spg_context* c) {
     operand_release(c->A);
     operand_release(c->B);
     DevBuf* bufs[] = {&c->ub, &c->ntile, &c->vstart, &c->vrow_row, &c->vlo, &c->vwid,
-                      &c->counters, &c->arow_ptr, &c->arow_k, &c->bsplit, &c->vub,
+                      &c->counters, &c->arow_ptr, &c->arow_k, &c->tilebits, &c->bsplit,
+                      &c->vub,
                       &c->vbase, &c->cursor, &c->kbin, &c->kperm, &c->kbin_off, &c->kitems,
                       &c->kitem_off, &c->icol, &c->ival, &c->vclass, &c->vlist,
                       &c->vclass_off, &c->vcnt, &c->vout, &c->crow, &c->ccol, &c->cval,
@@ -394,8 +395,8 @@ int spgInitGEMM(spg_context* c, cm* C, int64_t* nprod) {
     int64_t np = 0;
     SPG_TRY(read_scalar<int64_t>(c, (const int64_t*)c->vbase.p + c->V, &np));
     c->nprod = np;
-    SPG_TRY(ensure(c, c->icol, sizeof(int32_t) * (size_t)(np + 1)));
-    SPG_TRY(ensure(c, c->ival, sizeof(float) * (size_t)(np + 1)));
+    SPG_TRY(ensure(c, c->icol, sizeof(int32_t) * (size_t)(np + 16)));  // 16-byte loads read past the end
+    SPG_TRY(ensure(c, c->ival, sizeof(float) * (size_t)(np + 16)));
     tmark(c, SPG_PHASE_INIT, 1);
     if (C) {
         cmInit(C);
@@ -462,6 +463,7 @@ int spgSymbolicGEMM(spg_context* c, int64_t* nnz) {
     SPG_TRY(ensure(c, c->vclass_off, sizeof(int64_t) * (kNumVClasses + 1)));
     SPG_TRY(ensure(c, c->vcnt, sizeof(int64_t) * V1));
     SPG_TRY(ensure(c, c->vout, sizeof(int64_t) * (V1 + 1)));
+    SPG_TRY(ensure(c, c->tilebits, sizeof(uint32_t) * 256 * V1));
     SPG_TRY(launch_symbolic(c));
     int64_t total = 0;
     SPG_TRY(read_scalar<int64_t>(c, (const int64_t*)c->vout.p + c->V, &total));

@@ -890,13 +890,16 @@ __device__ __forceinline__ int hash_insert(int* keys, int key, bool& fresh) {
 // RANK: a row over the full column range (n <= 2^18) merged through a
 // bitmap of all columns whose prefix popcounts give each column its output
 // position (no sort).
-constexpr int kClassTile = kNumHashClasses;      // 8
-constexpr int kClassRank = kNumHashClasses + 1;  // 9
+constexpr int kClassTile = kNumHashClasses;       // 8: large tile rows (workgroup)
+constexpr int kClassRank = kNumHashClasses + 1;   // 9
+constexpr int kClassTileS = kNumHashClasses + 2;  // 10: tile rows, count <= 256 (wave)
+constexpr int kClassTileM = kNumHashClasses + 3;  // 11: tile rows, count <= 1024 (wave)
+constexpr int kTileWords = (1 << kTileMaxLog2) / 32;  // bitmap words of one tile
 constexpr int kRankMaxWords = 8192;              // bitmap of 2^18 columns
 constexpr int kRankMinCount = 128;               // smaller rows: hash + sort
 constexpr int kPT = 512;                          // threads of the persistent kernels
 
-static_assert(kNumVClasses >= kClassRank + 1, "class table too small");
+static_assert(kNumVClasses >= kClassTileM + 1, "class table too small");
 
 __global__ void k_vclass_symbolic(long long V, const long long* __restrict__ vub,
                                   const int* __restrict__ vwid, int W,
@@ -969,50 +972,72 @@ __device__ __forceinline__ T block_exscan_n(T x, T* total, T* sh) {
 
 // Symbolic count of tile-width virtual rows: one wave per row with a
 // wave-private bitmap of the tile's columns (no workgroup barriers, so the
-// 32 waves of a CU overlap their load latencies); four product loads are in
-// flight per lane before the LDS updates.
+// waves of a CU overlap their load latencies).  Products are read 16 bytes
+// per lane, the next row's descriptor is loaded while the current one is
+// counted, and the finished bitmap is stored for the merge pass.
 constexpr int kSymWaves = kPT / 64;
 
 __global__ __launch_bounds__(kPT) void k_symbolic_tile(
-    const int* __restrict__ vlist, long long nlist, unsigned long long* __restrict__ ctr,
-    const long long* __restrict__ vbase, const int* __restrict__ vlo,
-    const int* __restrict__ vwid, const int* __restrict__ icol, long long* __restrict__ vcnt) {
-    constexpr int NW = (1 << kTileMaxLog2) / 32;
-    __shared__ unsigned bitsall[kSymWaves][NW];
+    const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
+    const int* __restrict__ vlo, const int* __restrict__ vwid, const int* __restrict__ icol,
+    long long* __restrict__ vcnt, unsigned* __restrict__ tilebits) {
+    __shared__ unsigned bitsall[kSymWaves][kTileWords];
     const int wv = threadIdx.x >> 6, lane = lane_id();
     unsigned* bits = bitsall[wv];
-    for (int w = lane; w < NW; w += 64) bits[w] = 0u;
+    for (int w = lane; w < kTileWords; w += 64) bits[w] = 0u;
+    const long long stride = (long long)gridDim.x * kSymWaves;
+    long long j = (long long)blockIdx.x * kSymWaves + wv;
+    if (j >= nlist) return;
+    int v = vlist[j];
+    long long s0 = vbase[v], s1 = vbase[v + 1];
+    int lo = vlo[v];
     for (;;) {
-        long long j = 0;
-        if (lane == 0) j = (long long)atomicAdd(ctr, 1ull);
-        j = __shfl(j, 0, 64);
-        if (j >= nlist) break;
-        const int v = vlist[j];
-        const long long s0 = vbase[v], s1 = vbase[v + 1];
-        const int lo = vlo[v];
-        const int nw = (vwid[v] + 31) >> 5;
-        // trip count uniform across the wave (the OR below shuffles)
-        for (long long sb = s0; sb < s1; sb += 4 * 64) {
-            int c[4];
+        const long long jn = j + stride;
+        const int vn = jn < nlist ? vlist[jn] : -1;
+        for (long long sb = s0 & ~3LL; sb < s1; sb += 2 * 4 * 64) {
+            int4 q[2];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const long long q = sb + lane + u * 64;
-                c[u] = q < s1 ? icol[q] - lo : -1;
+            for (int u = 0; u < 2; ++u) {
+                const long long base = sb + (long long)(lane + u * 64) * 4;
+                q[u] = base < s1 ? *(const int4*)(icol + base) : make_int4(0, 0, 0, 0);
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) wave_bitmap_or(bits, c[u]);
+            for (int u = 0; u < 2; ++u) {
+                const long long base = sb + (long long)(lane + u * 64) * 4;
+                const int cc[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    if (base + e >= s0 && base + e < s1) {
+                        const int c = cc[e] - lo;
+                        atomicOr(&bits[c >> 5], 1u << (c & 31));
+                    }
+                }
+            }
+        }
+        long long s0n = 0, s1n = 0;
+        int lon = 0;
+        if (vn >= 0) {
+            s0n = vbase[vn];
+            s1n = vbase[vn + 1];
+            lon = vlo[vn];
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        int mine = 0;
-        for (int w = lane; w < nw; w += 64) {
-            mine += __popc(bits[w]);
-            bits[w] = 0u;
-        }
+        static_assert(kTileWords == 4 * 64, "four bitmap words per lane");
+        uint4 wq = *(const uint4*)(bits + lane * 4);
+        int mine = __popc(wq.x) + __popc(wq.y) + __popc(wq.z) + __popc(wq.w);
+        *(uint4*)(tilebits + (long long)v * kTileWords + lane * 4) = wq;
+        *(uint4*)(bits + lane * 4) = make_uint4(0u, 0u, 0u, 0u);
         mine = wave_sum(mine);
         if (lane == 0) vcnt[v] = mine;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
+        if (vn < 0) break;
+        j = jn;
+        v = vn;
+        s0 = s0n;
+        s1 = s1n;
+        lo = lon;
     }
 }
 
@@ -1053,7 +1078,6 @@ int launch_symbolic(spg_context* c) {
     long long off[kNumVClasses + 1];
     SPG_TRY(class_offsets(c, off));
     const int* vl = (const int*)c->vlist.p;
-    unsigned long long* ctr = (unsigned long long*)c->counters.p;
     tmark(c, SPG_PHASE_SYMBOLIC, 0);
     for (int b = 0; b < kNumVClasses; ++b) {
         const long long cnt = off[b + 1] - off[b];
@@ -1062,10 +1086,10 @@ int launch_symbolic(spg_context* c) {
         if (b == kClassTile) {
             hipLaunchKernelGGL(k_symbolic_tile,
                                dim3(persistent_grid(c, (cnt + kSymWaves - 1) / kSymWaves, 4)),
-                               dim3(kPT), 0,
-                               c->stream, list, cnt, ctr + 0, (const long long*)c->vbase.p,
+                               dim3(kPT), 0, c->stream, list, cnt, (const long long*)c->vbase.p,
                                (const int*)c->vlo.p, (const int*)c->vwid.p,
-                               (const int*)c->icol.p, (long long*)c->vcnt.p);
+                               (const int*)c->icol.p, (long long*)c->vcnt.p,
+                               (unsigned*)c->tilebits.p);
         } else {
             switch (b + kHashMinLog2) {
                 case 6: launch_sym_hash<6>(c, list, cnt); break;
@@ -1111,7 +1135,7 @@ __global__ void k_vclass_merge(long long V, const long long* __restrict__ vcnt,
     }
     const int width = vwid[v];
     if (width <= W) {
-        cls[v] = kClassTile;
+        cls[v] = cnt <= 256 ? kClassTileS : (cnt <= 1024 ? kClassTileM : kClassTile);
         return;
     }
     if (cnt > kRankMinCount && n <= kRankMaxWords * 32 && cnt <= kHashMaxCount) {
@@ -1240,29 +1264,54 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
     int v = vlist[j];
     long long s0 = vbase[v], s1 = vbase[v + 1], o0 = vout[v], lim = vout[v + 1];
     int lo = vlo[v], nw = (vwid[v] + 31) >> 5;
-    int c[4];
-    float x[4];
-    load4(icol, ival, s0 + threadIdx.x, s1, lo, c, x);
+    int4 qc = make_int4(0, 0, 0, 0);
+    float4 qv = make_float4(0.f, 0.f, 0.f, 0.f);
+    {
+        const long long b0 = (s0 & ~3LL) + (long long)threadIdx.x * 4;
+        if (b0 < s1) {
+            qc = *(const int4*)(icol + b0);
+            qv = *(const float4*)(ival + b0);
+        }
+    }
     for (;;) {
         const long long jn = j + G;
         const int vn = jn < nlist ? vlist[jn] : -1;
-        apply4(acc, bits, c, x);
-        // trip count uniform across the workgroup (apply4 shuffles)
-        for (long long sb = s0 + 4 * kPT; sb < s1; sb += 4 * kPT) {
-            load4(icol, ival, sb + threadIdx.x, s1, lo, c, x);
-            apply4(acc, bits, c, x);
+        // four consecutive products per thread and load (16-byte loads)
+        for (long long sb = s0 & ~3LL;;) {
+            const long long bb = sb + (long long)threadIdx.x * 4;
+            const int cc[4] = {qc.x, qc.y, qc.z, qc.w};
+            const float xx[4] = {qv.x, qv.y, qv.z, qv.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (bb + e >= s0 && bb + e < s1) {
+                    const int c = cc[e] - lo;
+                    atomicOr(&bits[c >> 5], 1u << (c & 31));
+                    unsafeAtomicAdd(&acc[c], (double)xx[e]);
+                }
+            }
+            sb += 4 * kPT;
+            if (sb >= s1) break;
+            const long long nb = sb + (long long)threadIdx.x * 4;
+            if (nb < s1) {
+                qc = *(const int4*)(icol + nb);
+                qv = *(const float4*)(ival + nb);
+            }
         }
         __syncthreads();
         long long s0n = 0, s1n = 0, o0n = 0, limn = 0;
         int lon = 0, nwn = 0;
-        if (vn >= 0) {
+        if (vn >= 0) {  // next row's descriptor and first products, before the emit
             s0n = vbase[vn];
             s1n = vbase[vn + 1];
             o0n = vout[vn];
             limn = vout[vn + 1];
             lon = vlo[vn];
             nwn = (vwid[vn] + 31) >> 5;
-            load4(icol, ival, s0n + threadIdx.x, s1n, lon, c, x);
+            const long long nb = (s0n & ~3LL) + (long long)threadIdx.x * 4;
+            if (nb < s1n) {
+                qc = *(const int4*)(icol + nb);
+                qv = *(const float4*)(ival + nb);
+            }
         }
         unsigned b = threadIdx.x < nw ? bits[threadIdx.x] : 0u;
         int tot;
@@ -1291,6 +1340,86 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
         lim = limn;
         lo = lon;
         nw = nwn;
+    }
+}
+
+// Tile-width virtual rows with at most CAP columns: one wave per row, no
+// workgroup barriers.  The row's column bitmap (stored by the symbolic
+// pass) and a prefix popcount per word give every column its rank, i.e.
+// its output position; products are accumulated in FP64 at acc[rank], the
+// column is recorded at col[rank], and the row is written out with fully
+// coalesced stores.
+template <int CAP, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
+    const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
+    const int* __restrict__ vlo, const unsigned* __restrict__ tilebits,
+    const int* __restrict__ icol, const float* __restrict__ ival,
+    const long long* __restrict__ vout, int* __restrict__ ccol, float* __restrict__ cval) {
+    __shared__ double accall[WPB][CAP];
+    __shared__ int colall[WPB][CAP];
+    __shared__ unsigned bitsall[WPB][kTileWords];
+    __shared__ unsigned short preall[WPB][kTileWords];
+    const int wv = threadIdx.x >> 6, lane = lane_id();
+    double* acc = accall[wv];
+    int* colr = colall[wv];
+    unsigned* bits = bitsall[wv];
+    unsigned short* pre = preall[wv];
+    const long long stride = (long long)gridDim.x * WPB;
+    for (long long j = (long long)blockIdx.x * WPB + wv; j < nlist; j += stride) {
+        const int v = vlist[j];
+        const long long s0 = vbase[v], s1 = vbase[v + 1];
+        const long long o0 = vout[v];
+        const int cnt = (int)min((long long)CAP, vout[v + 1] - o0);
+        const int lo = vlo[v];
+        const uint4 wq = *(const uint4*)(tilebits + (long long)v * kTileWords + lane * 4);
+        const int p0 = __popc(wq.x), p1 = __popc(wq.y), p2 = __popc(wq.z), p3 = __popc(wq.w);
+        const int mine = p0 + p1 + p2 + p3;
+        int inc = mine;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += y;
+        }
+        const int ex = inc - mine;
+        *(uint4*)(bits + lane * 4) = wq;
+        pre[lane * 4 + 0] = (unsigned short)ex;
+        pre[lane * 4 + 1] = (unsigned short)(ex + p0);
+        pre[lane * 4 + 2] = (unsigned short)(ex + p0 + p1);
+        pre[lane * 4 + 3] = (unsigned short)(ex + p0 + p1 + p2);
+        for (int q = lane; q < cnt; q += 64) acc[q] = 0.0;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (long long sb = s0 & ~3LL; sb < s1; sb += 4 * 64) {
+            const long long base = sb + (long long)lane * 4;
+            int4 qc = make_int4(0, 0, 0, 0);
+            float4 qv = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (base < s1) {
+                qc = *(const int4*)(icol + base);
+                qv = *(const float4*)(ival + base);
+            }
+            const int cc[4] = {qc.x, qc.y, qc.z, qc.w};
+            const float xx[4] = {qv.x, qv.y, qv.z, qv.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (base + e >= s0 && base + e < s1) {
+                    const int c = cc[e] - lo;
+                    const int w = c >> 5;
+                    const int r = pre[w] + __popc(bits[w] & ((1u << (c & 31)) - 1u));
+                    if (r < cnt) {
+                        unsafeAtomicAdd(&acc[r], (double)xx[e]);
+                        colr[r] = cc[e];
+                    }
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (int q = lane; q < cnt; q += 64) {
+            ccol[o0 + q] = colr[q];
+            cval[o0 + q] = (float)acc[q];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -1415,7 +1544,21 @@ int launch_merge(spg_context* c) {
         const long long cnt = off[b + 1] - off[b];
         if (cnt <= 0) continue;
         const int* list = vl + off[b];
-        if (b == kClassTile) {
+        if (b == kClassTileS) {
+            hipLaunchKernelGGL((k_merge_tile_wave<256, 8>),
+                               dim3(persistent_grid(c, (cnt + 7) / 8, 4)), dim3(512), 0, c->stream,
+                               list, cnt, (const long long*)c->vbase.p, (const int*)c->vlo.p,
+                               (const unsigned*)c->tilebits.p, (const int*)c->icol.p,
+                               (const float*)c->ival.p, (const long long*)c->vout.p,
+                               (int*)c->ccol.p, (float*)c->cval.p);
+        } else if (b == kClassTileM) {
+            hipLaunchKernelGGL((k_merge_tile_wave<1024, 4>),
+                               dim3(persistent_grid(c, (cnt + 3) / 4, 2)), dim3(256), 0, c->stream,
+                               list, cnt, (const long long*)c->vbase.p, (const int*)c->vlo.p,
+                               (const unsigned*)c->tilebits.p, (const int*)c->icol.p,
+                               (const float*)c->ival.p, (const long long*)c->vout.p,
+                               (int*)c->ccol.p, (float*)c->cval.p);
+        } else if (b == kClassTile) {
             hipLaunchKernelGGL(k_merge_tile, dim3(persistent_grid(c, cnt, 2)), dim3(kPT), 0,
                                c->stream, list, cnt, (const long long*)c->vbase.p,
                                (const int*)c->vlo.p, (const int*)c->vwid.p,

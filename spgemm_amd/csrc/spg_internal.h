@@ -19,7 +19,7 @@ constexpr int kTileMaxLog2 = 13;        // dense tile: 8192 FP64 accumulators
 constexpr int kHashMinLog2 = 6;         // smallest hash: 64 slots
 // Merge/symbolic classes of a virtual row.
 constexpr int kNumHashClasses = kHashMaxLog2 - kHashMinLog2 + 1;  // 64..8192
-constexpr int kNumVClasses = kNumHashClasses + 2;  // + tile, rank
+constexpr int kNumVClasses = kNumHashClasses + 4;  // + tile (large), rank, tile small/medium
 
 // Grow-only device buffer.
 struct DevBuf {
@@ -69,6 +69,7 @@ struct spg_context {
     spg::DevBuf ub, ntile, vstart, vrow_row, vlo, vwid, bsplit, vub, vbase, cursor;
     spg::DevBuf counters;  // work counters of persistent kernels
     spg::DevBuf arow_ptr, arow_k;  // A transposed: row i -> its column indices k
+    spg::DevBuf tilebits;          // column bitmap of every tile-width virtual row
     int num_cu = 256;
     // categorize products
     spg::DevBuf kbin, kperm, kbin_off, kitems, kitem_off;
