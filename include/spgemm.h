@@ -169,7 +169,15 @@ typedef struct spg_config {
     int64_t long_row_ub;       /* rows with more products are column-tiled     */
     int32_t xcd_swizzle;       /* 1: map consecutive work items to one XCD     */
     int32_t timing;            /* 1: record per-phase HIP events               */
+    int32_t expand_mode;       /* SPG_EXPAND_KBINS or SPG_EXPAND_ROWS          */
 } spg_config;
+
+/* Expansion mappings of BinGEMM.  KBINS groups the (i,k) pairs by k as the
+ * reference does (one work item per k-chunk, k binned by nnz(B row k);
+ * src/mm.cu:288-327); ROWS groups them by C row (virtual row), so that each
+ * row's slot range is written front to back with deterministic offsets. */
+#define SPG_EXPAND_KBINS 0
+#define SPG_EXPAND_ROWS 1
 
 int spgCreate(spg_context** ctx, int device);
 int spgDestroy(spg_context* ctx);

@@ -33,6 +33,8 @@ OPERAND_B = 1
 NUM_KBINS = 9
 PHASES = ("init", "categorize", "expand", "symbolic", "merge")
 MTX_COMPAT_VALUES = 1
+EXPAND_KBINS = 0
+EXPAND_ROWS = 1
 
 STATUS = {0: "SPG_OK", 1: "SPG_EINVAL", 2: "SPG_ENOMEM", 3: "SPG_EHIP", 4: "SPG_EORDER",
           5: "SPG_EIO", 6: "SPG_ENODEV", 7: "SPG_EOVERFLOW"}
@@ -65,7 +67,7 @@ class CSR64(Structure):
 
 class Config(Structure):
     _fields_ = [("tile_log2", c_int32), ("item_products", c_int32), ("long_row_ub", c_int64),
-                ("xcd_swizzle", c_int32), ("timing", c_int32)]
+                ("xcd_swizzle", c_int32), ("timing", c_int32), ("expand_mode", c_int32)]
 
 
 TRIPLET_DTYPE = np.dtype([("ridx", np.int32), ("cidx", np.int32), ("val", np.float32)])

@@ -145,6 +145,7 @@ int spgCreate(spg_context** out, int device) {
     c->cfg.long_row_ub = kHashMaxCount;
     c->cfg.xcd_swizzle = 1;
     c->cfg.timing = 1;
+    c->cfg.expand_mode = SPG_EXPAND_ROWS;
     int rc = SPG_OK;
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -177,7 +178,8 @@ int spgDestroy(spg_context* c) {
     operand_release(c->A);
     operand_release(c->B);
     DevBuf* bufs[] = {&c->ub, &c->ntile, &c->vstart, &c->vrow_row, &c->vlo, &c->vwid,
-                      &c->counters, &c->arow_ptr, &c->arow_k, &c->tilebits, &c->bsplit,
+                      &c->counters, &c->arow_ptr, &c->arow_k, &c->arow_v, &c->tilebits,
+                      &c->bsplit,
                       &c->vub,
                       &c->vbase, &c->cursor, &c->kbin, &c->kperm, &c->kbin_off, &c->kitems,
                       &c->kitem_off, &c->icol, &c->ival, &c->vclass, &c->vlist,
@@ -215,6 +217,8 @@ int spgSetConfig(spg_context* c, const spg_config* cfg) {
         return fail(c, SPG_EINVAL, "item_products must be in [64, 2^20]");
     if (cfg->long_row_ub < 1 || cfg->long_row_ub > kHashMaxCount)
         return fail(c, SPG_EINVAL, "long_row_ub must be in [1, 4096]");
+    if (cfg->expand_mode != SPG_EXPAND_KBINS && cfg->expand_mode != SPG_EXPAND_ROWS)
+        return fail(c, SPG_EINVAL, "expand_mode must be SPG_EXPAND_KBINS or SPG_EXPAND_ROWS");
     c->cfg = *cfg;
     c->stage = std::min(c->stage, 0);
     return SPG_OK;
@@ -376,6 +380,7 @@ int spgInitGEMM(spg_context* c, cm* C, int64_t* nprod) {
     SPG_TRY(ensure(c, c->vstart, sizeof(int32_t) * ((size_t)c->m + 1)));
     SPG_TRY(ensure(c, c->arow_ptr, sizeof(int32_t) * ((size_t)c->m + 1)));
     SPG_TRY(ensure(c, c->arow_k, sizeof(int32_t) * ((size_t)c->A.nnz + 1)));
+    SPG_TRY(ensure(c, c->arow_v, sizeof(float) * ((size_t)c->A.nnz + 1)));
     SPG_TRY(launch_init(c));
     int32_t V = 0;
     SPG_TRY(read_scalar<int32_t>(c, (const int32_t*)c->vstart.p + c->m, &V));
@@ -388,6 +393,9 @@ int spgInitGEMM(spg_context* c, cm* C, int64_t* nprod) {
     SPG_TRY(ensure(c, c->vub, sizeof(int64_t) * ((size_t)c->V + 1)));
     SPG_TRY(ensure(c, c->vbase, sizeof(int64_t) * ((size_t)c->V + 1)));
     SPG_TRY(ensure(c, c->cursor, sizeof(int32_t) * ((size_t)c->V + 1)));
+    SPG_TRY(ensure(c, c->vclass, sizeof(int32_t) * ((size_t)c->V + 1)));
+    SPG_TRY(ensure(c, c->vlist, sizeof(int32_t) * ((size_t)c->V + 1)));
+    SPG_TRY(ensure(c, c->vclass_off, sizeof(int64_t) * (kNumVClasses + 1)));
     if (c->have_long)
         SPG_TRY(ensure(c, c->bsplit, sizeof(int32_t) * (size_t)c->K * (size_t)(c->T + 1)));
     SPG_TRY(scratch_prepare(c, std::max<long long>(c->V, c->K) + 1));
@@ -438,11 +446,15 @@ static int expand_common(spg_context* c, bool binned) {
         return fail(c, SPG_EORDER, binned ? "CategorizeGEMM must run first"
                                           : "InitGEMM must run first");
     SPG_HIP(hipSetDevice(c->device));
-    SPG_TRY(scratch_prepare(c, 1));
-    SPG_TRY(spg::fill<int>(c, (int*)c->cursor.p, 0, c->V));
-    tmark(c, SPG_PHASE_EXPAND, 0);
-    SPG_TRY(binned ? launch_expand_binned(c) : launch_expand_simple(c));
-    tmark(c, SPG_PHASE_EXPAND, 1);
+    SPG_TRY(scratch_prepare(c, c->V + 1));
+    if (binned && c->cfg.expand_mode == SPG_EXPAND_ROWS) {
+        SPG_TRY(launch_expand_rows(c));  // records its own timer around the kernels
+    } else {
+        SPG_TRY(spg::fill<int>(c, (int*)c->cursor.p, 0, c->V));
+        tmark(c, SPG_PHASE_EXPAND, 0);
+        SPG_TRY(binned ? launch_expand_binned(c) : launch_expand_simple(c));
+        tmark(c, SPG_PHASE_EXPAND, 1);
+    }
     c->stage = 3;
     return SPG_OK;
 }

@@ -317,13 +317,19 @@ __global__ __launch_bounds__(kBlock) void k_row_hist(const int* __restrict__ a_p
 }
 
 __global__ __launch_bounds__(kBlock) void k_row_fill(const int* __restrict__ a_ptr,
-                                                      const int* __restrict__ a_idx, int K,
+                                                      const int* __restrict__ a_idx,
+                                                      const float* __restrict__ a_val, int K,
                                                       int* __restrict__ rcur,
-                                                      int* __restrict__ rk) {
+                                                      int* __restrict__ rk,
+                                                      float* __restrict__ rv) {
     const long long k = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
     if (k >= K) return;
     const int e1 = a_ptr[k + 1];
-    for (int e = a_ptr[k] + lane_id(); e < e1; e += 64) rk[atomicAdd(&rcur[a_idx[e]], 1)] = (int)k;
+    for (int e = a_ptr[k] + lane_id(); e < e1; e += 64) {
+        const int pos = atomicAdd(&rcur[a_idx[e]], 1);
+        rk[pos] = (int)k;
+        rv[pos] = a_val[e];
+    }
 }
 
 template <typename T>
@@ -468,7 +474,7 @@ int launch_init(spg_context* c) {
     SPG_HIP(hipGetLastError());
     if (K > 0) {
         hipLaunchKernelGGL(k_row_fill, dim3((unsigned)nbk), dim3(kBlock), 0, c->stream, c->A.ptr,
-                           c->A.idx, K, rcnt, rk);
+                           c->A.idx, c->A.val, K, rcnt, rk, (float*)c->arow_v.p);
         SPG_HIP(hipGetLastError());
     }
     if (m > 0) {
@@ -859,6 +865,203 @@ __device__ __forceinline__ void wave_bitmap_or(unsigned* bits, int c) {
     if (c >= 0 && (lane == 63 || wn != w)) atomicOr(&bits[w], m);
 }
 
+// ---------------------------------------------- row-grouped expansion
+// The same (i,k) products as the column-grouped kernels, grouped by C row:
+// the work unit is one virtual row, whose slot range is filled front to
+// back.  For every k of row i (A transposed to rows) the piece of B row k
+// inside the virtual row's column range is scaled by a_ik; piece offsets
+// are an exclusive scan of the piece lengths, so no slot is reserved with
+// atomics and every store of a wave lands in one contiguous run (HBM write
+// efficiency depends on run length; scattered 128-byte runs reach only a
+// fraction of the sequential write bandwidth).  A lane finds the piece of
+// its output position by a binary search over the 64 piece offsets.
+constexpr long long kRowsWaveMax = 16384;  // larger virtual rows: workgroup kernel
+
+struct RowsArgs {
+    const int* rptr;
+    const int* rk;
+    const float* rv;
+    const int* b_ptr;
+    const int* b_idx;
+    const float* b_val;
+    const int* bsplit;
+    const int* vrow_row;
+    const int* vstart;
+    const long long* vbase;
+    int T;
+    int* icol;
+    float* ival;
+};
+
+// Piece [ps, ps+len) of B row k inside tile t (whole row when untiled).
+__device__ __forceinline__ void piece_of(const RowsArgs& g, int k, int t, bool tiled, int& ps,
+                                         int& len) {
+    if (tiled) {
+        const int* bs = g.bsplit + (long long)k * (g.T + 1);
+        ps = bs[t];
+        len = bs[t + 1] - ps;
+    } else {
+        ps = g.b_ptr[k];
+        len = g.b_ptr[k + 1] - ps;
+    }
+}
+
+__global__ __launch_bounds__(kPT) void k_expand_rows_wave(RowsArgs g, const int* __restrict__ vlist,
+                                                           long long nlist) {
+    const int wv = threadIdx.x >> 6, lane = lane_id();
+    constexpr int NWV = kPT / 64;
+    const long long stride = (long long)gridDim.x * NWV;
+    const int* __restrict__ bidx = g.b_idx;
+    const float* __restrict__ bval = g.b_val;
+    int* __restrict__ icol = g.icol;
+    float* __restrict__ ival = g.ival;
+    for (long long j = (long long)blockIdx.x * NWV + wv; j < nlist; j += stride) {
+        const int v = vlist[j];
+        const int i = g.vrow_row[v];
+        const int vs = g.vstart[i];
+        const bool tiled = g.vstart[i + 1] - vs > 1;
+        const int t = v - vs;
+        const int e1 = g.rptr[i + 1];
+        long long dst = g.vbase[v];
+        for (int c0 = g.rptr[i]; c0 < e1; c0 += 64) {
+            const int e = c0 + lane;
+            int ps = 0, len = 0;
+            float a = 0.f;
+            if (e < e1) {
+                piece_of(g, g.rk[e], t, tiled, ps, len);
+                a = g.rv[e];
+            }
+            int inc = len;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += y;
+            }
+            const int off = inc - len;
+            const int total = __shfl(inc, 63, 64);
+            for (int pb = 0; pb < total; pb += 64) {
+                const int p = pb + lane;
+                int lo = 0;  // largest lane whose piece starts at or before p
+#pragma unroll
+                for (int step = 32; step > 0; step >>= 1) {
+                    const int cand = lo + step;
+                    const int oc = __shfl(off, cand, 64);
+                    if (oc <= p) lo = cand;
+                }
+                const int src_ps = __shfl(ps, lo, 64);
+                const int src_off = __shfl(off, lo, 64);
+                const float src_a = __shfl(a, lo, 64);
+                if (p < total) {
+                    const int src = src_ps + (p - src_off);
+                    icol[dst + p] = bidx[src];
+                    ival[dst + p] = __fmul_rn(src_a, bval[src]);
+                }
+            }
+            dst += total;
+        }
+    }
+}
+
+// Virtual rows with more than kRowsWaveMax products: one workgroup each,
+// k taken kBlock at a time, offsets scanned across the workgroup and the
+// piece of each output found by a binary search in LDS.
+__global__ __launch_bounds__(kBlock) void k_expand_rows_wg(RowsArgs g, const int* __restrict__ vlist,
+                                                            long long nlist) {
+    __shared__ int soff[kBlock + 1];
+    __shared__ int sps[kBlock];
+    __shared__ float sa[kBlock];
+    __shared__ int sh[kBlock / 64];
+    for (long long j = blockIdx.x; j < nlist; j += gridDim.x) {
+        const int v = vlist[j];
+        const int i = g.vrow_row[v];
+        const int vs = g.vstart[i];
+        const bool tiled = g.vstart[i + 1] - vs > 1;
+        const int t = v - vs;
+        const int e1 = g.rptr[i + 1];
+        long long dst = g.vbase[v];
+        for (int c0 = g.rptr[i]; c0 < e1; c0 += kBlock) {
+            const int e = c0 + threadIdx.x;
+            int ps = 0, len = 0;
+            float a = 0.f;
+            if (e < e1) {
+                piece_of(g, g.rk[e], t, tiled, ps, len);
+                a = g.rv[e];
+            }
+            int total;
+            const int off = block_exscan<int>(len, &total, sh);
+            soff[threadIdx.x] = off;
+            sps[threadIdx.x] = ps;
+            sa[threadIdx.x] = a;
+            if (threadIdx.x == 0) soff[kBlock] = total;
+            __syncthreads();
+            for (int p = threadIdx.x; p < total; p += kBlock) {
+                int lo = 0;
+#pragma unroll
+                for (int step = kBlock / 2; step > 0; step >>= 1)
+                    if (soff[lo + step] <= p) lo += step;
+                const int src = sps[lo] + (p - soff[lo]);
+                g.icol[dst + p] = g.b_idx[src];
+                g.ival[dst + p] = __fmul_rn(sa[lo], g.b_val[src]);
+            }
+            __syncthreads();
+            dst += total;
+        }
+    }
+}
+
+__global__ void k_vclass_expand(long long V, const long long* __restrict__ vub,
+                                int* __restrict__ cls) {
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const long long u = vub[v];
+    cls[v] = u == 0 ? -1 : (u <= kRowsWaveMax ? 0 : 1);
+}
+
+int launch_expand_rows(spg_context* c) {
+    const long long V = c->V;
+    if (V <= 0) return SPG_OK;
+    hipLaunchKernelGGL(k_vclass_expand, dim3((unsigned)((V + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       c->stream, V, (const long long*)c->vub.p, (int*)c->vclass.p);
+    SPG_HIP(hipGetLastError());
+    SPG_TRY(stable_partition(c, (const int32_t*)c->vclass.p, V, 2, (int32_t*)c->vlist.p,
+                             (int64_t*)c->vclass_off.p));
+    long long* off_h = (long long*)c->pinned_small.p;
+    SPG_HIP(hipMemcpyAsync(off_h, c->vclass_off.p, sizeof(long long) * 3, hipMemcpyDeviceToHost,
+                           c->stream));
+    SPG_HIP(hipStreamSynchronize(c->stream));
+    const long long n0 = off_h[1] - off_h[0], n1 = off_h[2] - off_h[1];
+    RowsArgs g{};
+    g.rptr = (const int*)c->arow_ptr.p;
+    g.rk = (const int*)c->arow_k.p;
+    g.rv = (const float*)c->arow_v.p;
+    g.b_ptr = c->B.ptr;
+    g.b_idx = c->B.idx;
+    g.b_val = c->B.val;
+    g.bsplit = (const int*)c->bsplit.p;
+    g.vrow_row = (const int*)c->vrow_row.p;
+    g.vstart = (const int*)c->vstart.p;
+    g.vbase = (const long long*)c->vbase.p;
+    g.T = c->T;
+    g.icol = (int*)c->icol.p;
+    g.ival = (float*)c->ival.p;
+    const int* vl = (const int*)c->vlist.p;
+    tmark(c, SPG_PHASE_EXPAND, 0);
+    if (n1 > 0) {  // largest rows first
+        hipLaunchKernelGGL(k_expand_rows_wg, dim3(persistent_grid(c, n1, 8)), dim3(kBlock), 0,
+                           c->stream, g, vl + off_h[1], n1);
+        SPG_HIP(hipGetLastError());
+    }
+    if (n0 > 0) {
+        hipLaunchKernelGGL(k_expand_rows_wave,
+                           dim3(persistent_grid(c, (n0 + kPT / 64 - 1) / (kPT / 64), 4)), dim3(kPT),
+                           0, c->stream, g, vl + off_h[0], n0);
+        SPG_HIP(hipGetLastError());
+    }
+    tmark(c, SPG_PHASE_EXPAND, 1);
+    return SPG_OK;
+}
+
+
 __device__ __forceinline__ unsigned hash_slot(int key, int log2h) {
     return ((unsigned)key * 2654435761u) >> (32 - log2h);
 }
@@ -897,7 +1100,6 @@ constexpr int kClassTileM = kNumHashClasses + 3;  // 11: tile rows, count <= 102
 constexpr int kTileWords = (1 << kTileMaxLog2) / 32;  // bitmap words of one tile
 constexpr int kRankMaxWords = 8192;              // bitmap of 2^18 columns
 constexpr int kRankMinCount = 128;               // smaller rows: hash + sort
-constexpr int kPT = 512;                          // threads of the persistent kernels
 
 static_assert(kNumVClasses >= kClassTileM + 1, "class table too small");
 

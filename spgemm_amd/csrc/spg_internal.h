@@ -13,6 +13,7 @@ namespace spg {
 
 // Upper limits of the merge structures held in LDS.
 constexpr int kBlock = 256;             // threads per workgroup
+constexpr int kPT = 512;                // threads of the persistent kernels
 constexpr int kHashMaxLog2 = 13;        // largest LDS hash: 8192 slots
 constexpr int kHashMaxCount = 1 << (kHashMaxLog2 - 1);  // <= 4096 keys
 constexpr int kTileMaxLog2 = 13;        // dense tile: 8192 FP64 accumulators
@@ -68,7 +69,7 @@ struct spg_context {
     // init products
     spg::DevBuf ub, ntile, vstart, vrow_row, vlo, vwid, bsplit, vub, vbase, cursor;
     spg::DevBuf counters;  // work counters of persistent kernels
-    spg::DevBuf arow_ptr, arow_k;  // A transposed: row i -> its column indices k
+    spg::DevBuf arow_ptr, arow_k, arow_v;  // A transposed: row i -> its (k, a_ik)
     spg::DevBuf tilebits;          // column bitmap of every tile-width virtual row
     int num_cu = 256;
     // categorize products
@@ -117,6 +118,7 @@ int launch_init(spg_context* c);
 int launch_categorize(spg_context* c);
 int launch_expand_binned(spg_context* c);
 int launch_expand_simple(spg_context* c);
+int launch_expand_rows(spg_context* c);
 int launch_symbolic(spg_context* c);
 int launch_merge(spg_context* c);
 int launch_regroup_intermediate(spg_context* c, int64_t* d_rowptr);

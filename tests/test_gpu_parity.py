@@ -19,7 +19,8 @@ ZERO_ATOL = 1e-30    # exact cancellations: both sides must be ~0
 
 
 def run_gpu(spg, ctx, A, Ash, B, Bsh, simple=False, via_coo=True, cfg=None):
-    base_cfg = dict(tile_log2=13, item_products=4096, long_row_ub=4096, xcd_swizzle=1, timing=1)
+    base_cfg = dict(tile_log2=13, item_products=4096, long_row_ub=4096, xcd_swizzle=1, timing=1,
+                    expand_mode=spg.EXPAND_ROWS)
     base_cfg.update(cfg or {})
     ctx.set_config(**base_cfg)
     m, K = Ash
@@ -130,11 +131,12 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("mode", [0, 1], ids=["kbins", "rows"])
 @pytest.mark.parametrize("label,kind,kw", CASES, ids=[c[0] for c in CASES])
-def test_square_products(spg, orc, gpu_ctx, label, kind, kw):
-    print(f"case {label}", flush=True)
+def test_square_products(spg, orc, gpu_ctx, label, kind, kw, mode):
+    print(f"case {label} mode={mode}", flush=True)
     A, sh = gen(spg, kind, **kw)
-    out = run_gpu(spg, gpu_ctx, A, sh, A, sh)
+    out = run_gpu(spg, gpu_ctx, A, sh, A, sh, cfg=dict(expand_mode=mode))
     match, e64 = compare(out, oracle_all(orc, A, sh, A, sh))
     print(f"  exact-match fraction {match:.6f}, max |c-c64|/sum|ab| {e64:.3e}")
 
@@ -148,13 +150,16 @@ def test_rectangular(spg, orc, gpu_ctx):
     compare(run_gpu(spg, gpu_ctx, A, Ash, B, Bsh, via_coo=False), oracle_all(orc, A, Ash, B, Bsh))
 
 
+@pytest.mark.parametrize("mode", [0, 1], ids=["kbins", "rows"])
 @pytest.mark.parametrize("tile_log2,long_ub", [(6, 16), (8, 64), (5, 1)])
-def test_long_row_tiling(spg, orc, gpu_ctx, tile_log2, long_ub):
+def test_long_row_tiling(spg, orc, gpu_ctx, tile_log2, long_ub, mode):
     """Small tiles force the column-tiled path (bsplit, per-tile slots,
-    bitmap symbolic pass, dense merge) on a size the oracle checks."""
-    print(f"case tiling tile_log2={tile_log2} long_ub={long_ub}", flush=True)
+    bitmap symbolic pass, dense merge) on a size the oracle checks, for both
+    expansion mappings."""
+    print(f"case tiling tile_log2={tile_log2} long_ub={long_ub} mode={mode}", flush=True)
     A, sh = gen(spg, "rmat", s=10, ef=16, seed=7)
-    out = run_gpu(spg, gpu_ctx, A, sh, A, sh, cfg=dict(tile_log2=tile_log2, long_row_ub=long_ub))
+    out = run_gpu(spg, gpu_ctx, A, sh, A, sh,
+                  cfg=dict(tile_log2=tile_log2, long_row_ub=long_ub, expand_mode=mode))
     compare(out, oracle_all(orc, A, sh, A, sh))
 
 
@@ -174,7 +179,8 @@ def test_work_item_shapes(spg, orc, gpu_ctx, item_products, swz):
     """Small work items split long B rows across items (2D split of heavy k)."""
     print(f"case items P={item_products} swizzle={swz}", flush=True)
     A, sh = gen(spg, "rmat", s=11, ef=16, seed=8)
-    out = run_gpu(spg, gpu_ctx, A, sh, A, sh, cfg=dict(item_products=item_products, xcd_swizzle=swz))
+    out = run_gpu(spg, gpu_ctx, A, sh, A, sh,
+                  cfg=dict(item_products=item_products, xcd_swizzle=swz, expand_mode=spg.EXPAND_KBINS))
     compare(out, oracle_all(orc, A, sh, A, sh))
 
 
@@ -242,7 +248,8 @@ def test_reference_call_order_without_symbolic(spg, orc, gpu_ctx):
     print("case reference call order", flush=True)
     A, sh = gen(spg, "stencil", L=10, seed=3)
     ctx = gpu_ctx
-    ctx.set_config(tile_log2=13, item_products=4096, long_row_ub=4096, xcd_swizzle=1, timing=1)
+    ctx.set_config(tile_log2=13, item_products=4096, long_row_ub=4096, xcd_swizzle=1, timing=1,
+                   expand_mode=spg.EXPAND_ROWS)
     ptrA = ctx.coo2cm(spg.HostCOO(*A, *sh).sorted(spg.COL_MAJOR), return_host=True)
     ptrB = ctx.coo2cm(spg.HostCOO(*A, *sh).sorted(spg.ROW_MAJOR), return_host=True)
     Acsc, Bcsr = csc_of(A, sh), csr_of(A, sh)
