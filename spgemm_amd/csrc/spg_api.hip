@@ -141,11 +141,11 @@ int spgCreate(spg_context** out, int device) {
     if (!c) return SPG_ENOMEM;
     c->device = device;
     c->cfg.tile_log2 = kTileMaxLog2;
-    c->cfg.item_products = 4096;
+    c->cfg.item_products = 8192;
     c->cfg.long_row_ub = kHashMaxCount;
     c->cfg.xcd_swizzle = 1;
     c->cfg.timing = 1;
-    c->cfg.expand_mode = SPG_EXPAND_ROWS;
+    c->cfg.expand_mode = SPG_EXPAND_KBINS;
     int rc = SPG_OK;
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {

@@ -939,22 +939,44 @@ __global__ __launch_bounds__(kPT) void k_expand_rows_wave(RowsArgs g, const int*
             }
             const int off = inc - len;
             const int total = __shfl(inc, 63, 64);
-            for (int pb = 0; pb < total; pb += 64) {
-                const int p = pb + lane;
-                int lo = 0;  // largest lane whose piece starts at or before p
+            // four independent output groups per iteration keep four gathers
+            // in flight per lane
+            for (int pb = 0; pb < total; pb += 4 * 64) {
+                int p[4], lo[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    p[u] = pb + u * 64 + lane;
+                    lo[u] = 0;  // largest lane whose piece starts at or before p
+                }
 #pragma unroll
                 for (int step = 32; step > 0; step >>= 1) {
-                    const int cand = lo + step;
-                    const int oc = __shfl(off, cand, 64);
-                    if (oc <= p) lo = cand;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int cand = lo[u] + step;
+                        const int oc = __shfl(off, cand, 64);
+                        if (oc <= p[u]) lo[u] = cand;
+                    }
                 }
-                const int src_ps = __shfl(ps, lo, 64);
-                const int src_off = __shfl(off, lo, 64);
-                const float src_a = __shfl(a, lo, 64);
-                if (p < total) {
-                    const int src = src_ps + (p - src_off);
-                    icol[dst + p] = bidx[src];
-                    ival[dst + p] = __fmul_rn(src_a, bval[src]);
+                int src[4];
+                float sa[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    src[u] = __shfl(ps, lo[u], 64) + (p[u] - __shfl(off, lo[u], 64));
+                    sa[u] = __shfl(a, lo[u], 64);
+                }
+                int col[4];
+                float bv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    col[u] = p[u] < total ? bidx[src[u]] : 0;
+                    bv[u] = p[u] < total ? bval[src[u]] : 0.f;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (p[u] < total) {
+                        icol[dst + p[u]] = col[u];
+                        ival[dst + p[u]] = __fmul_rn(sa[u], bv[u]);
+                    }
                 }
             }
             dst += total;
@@ -994,14 +1016,34 @@ __global__ __launch_bounds__(kBlock) void k_expand_rows_wg(RowsArgs g, const int
             sa[threadIdx.x] = a;
             if (threadIdx.x == 0) soff[kBlock] = total;
             __syncthreads();
-            for (int p = threadIdx.x; p < total; p += kBlock) {
-                int lo = 0;
+            for (int pb = 0; pb < total; pb += 4 * kBlock) {
+                int p[4], lo[4];
 #pragma unroll
-                for (int step = kBlock / 2; step > 0; step >>= 1)
-                    if (soff[lo + step] <= p) lo += step;
-                const int src = sps[lo] + (p - soff[lo]);
-                g.icol[dst + p] = g.b_idx[src];
-                g.ival[dst + p] = __fmul_rn(sa[lo], g.b_val[src]);
+                for (int u = 0; u < 4; ++u) {
+                    p[u] = pb + u * kBlock + threadIdx.x;
+                    lo[u] = 0;
+                }
+#pragma unroll
+                for (int step = kBlock / 2; step > 0; step >>= 1) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (soff[lo[u] + step] <= p[u]) lo[u] += step;
+                }
+                int col[4];
+                float bv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int src = sps[lo[u]] + (p[u] - soff[lo[u]]);
+                    col[u] = p[u] < total ? g.b_idx[src] : 0;
+                    bv[u] = p[u] < total ? g.b_val[src] : 0.f;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (p[u] < total) {
+                        g.icol[dst + p[u]] = col[u];
+                        g.ival[dst + p[u]] = __fmul_rn(sa[lo[u]], bv[u]);
+                    }
+                }
             }
             __syncthreads();
             dst += total;
