@@ -97,7 +97,9 @@ typedef struct spg_csr64 {
 #define SPG_PHASE_EXPAND 2
 #define SPG_PHASE_SYMBOLIC 3
 #define SPG_PHASE_MERGE 4
-#define SPG_NUM_PHASES 5
+#define SPG_PHASE_PREDICT 5   /* learned path: inference + capacities       */
+#define SPG_PHASE_COMPACT 6   /* learned path: padded rows -> CSR           */
+#define SPG_NUM_PHASES 7
 
 /* readMTX flags. */
 #define SPG_MTX_COMPAT_VALUES 1  /* every value := 1.1 (src/coo.cpp:151-168) */
@@ -170,6 +172,8 @@ typedef struct spg_config {
     int32_t xcd_swizzle;       /* 1: map consecutive work items to one XCD     */
     int32_t timing;            /* 1: record per-phase HIP events               */
     int32_t expand_mode;       /* SPG_EXPAND_KBINS or SPG_EXPAND_ROWS          */
+    float pred_margin;         /* learned path: capacity = pred*(1+margin)+slack */
+    int32_t pred_slack;        /* learned path: slots added to every capacity    */
 } spg_config;
 
 /* Expansion mappings of BinGEMM.  KBINS groups the (i,k) pairs by k as the
@@ -246,6 +250,59 @@ int spgGetIntermediateByRow(spg_context* ctx, int64_t* row_ptr,
 /* Milliseconds per phase (SPG_NUM_PHASES) of the last product, from HIP
  * events on the context stream; requires cfg.timing. */
 int spgGetPhaseTimes(spg_context* ctx, double* ms);
+
+/* ---------------- learned allocation (no reference counterpart) ---------
+ * A 4-64-64-1 FP32 MLP predicts nnz(C row i) from four features of the
+ * row: log1p of nnz(A row i), of ub_i, of max_k nnz(B row k) and of the
+ * standard deviation of nnz(B row k) over the row's k.  The learned merge
+ * sizes every row's output from the prediction instead of the symbolic
+ * pass, detects rows that did not fit, merges those again with their upper
+ * bound as capacity (exact fallback), and compacts the rows into CSR.  The
+ * result is identical to spgMergeGEMM's. */
+typedef struct spg_predictor spg_predictor;
+
+typedef struct spg_learned_stats {
+    int64_t vrows;           /* virtual rows with at least one product        */
+    int64_t overflow_vrows;  /* rows whose prediction was too small (re-merged) */
+    int64_t unrecovered;     /* rows still not fitting after the fallback (0)  */
+    int64_t padded_slots;    /* sum of predicted capacities                    */
+    int64_t fallback_slots;  /* sum of fallback capacities                     */
+    int64_t nnz;             /* nnz(C)                                         */
+} spg_learned_stats;
+
+int spgPredictorCreate(spg_predictor** p, int device, uint64_t seed);
+int spgPredictorDestroy(spg_predictor* p);
+const char* spgPredictorGetLastError(spg_predictor* p);
+int spgPredictorNumParams(void);
+/* Adds the non-empty rows of ctx's last product as training samples
+ * (features and exact counts); needs SymbolicGEMM or a merge on ctx. */
+int spgPredictorAddSamples(spg_predictor* p, spg_context* ctx);
+/* Adam on the mean squared error of log1p(cnt) - log1p(ub) over all
+ * samples; batch is rounded down to a multiple of 64.  Deterministic.  The
+ * feature normalization is fixed from the samples at the first call. */
+int spgPredictorTrain(spg_predictor* p, int epochs, int batch, float lr,
+                      float* final_loss);
+/* Mean squared error over all samples and its gradient (NumParams floats)
+ * at the current weights; no update.  For checking the training kernels. */
+int spgPredictorGradient(spg_predictor* p, float* grad, float* loss);
+/* w: spgPredictorNumParams() floats; norm8: feature means then sds. */
+int spgPredictorGetWeights(spg_predictor* p, float* w, float* norm8);
+int spgPredictorSetWeights(spg_predictor* p, const float* w, const float* norm8);
+int spgPredictorSave(spg_predictor* p, const char* path);
+int spgPredictorLoad(spg_predictor* p, const char* path);
+
+/* Predicted nnz of every C row of ctx's product (after InitGEMM); kept on
+ * the context for spgLearnedMergeGEMM.  pred (may be NULL) receives m floats. */
+int spgPredictRows(spg_context* ctx, const spg_predictor* p, float* pred);
+/* Replaces the context's row predictions with m caller values (testing and
+ * external predictors). */
+int spgSetRowPredictions(spg_context* ctx, const float* pred);
+/* Merge with predicted capacities (after BinGEMM/SimpleGEMM and
+ * spgPredictRows/spgSetRowPredictions); st may be NULL. */
+int spgLearnedMergeGEMM(spg_context* ctx, spg_learned_stats* st);
+/* InitGEMM, CategorizeGEMM, BinGEMM, PredictRows, LearnedMergeGEMM. */
+int spgMultiplyLearned(spg_context* ctx, const spg_predictor* p,
+                       spg_learned_stats* st);
 
 #ifdef __cplusplus
 }

@@ -31,7 +31,7 @@ COL_MAJOR = 1
 OPERAND_A = 0
 OPERAND_B = 1
 NUM_KBINS = 9
-PHASES = ("init", "categorize", "expand", "symbolic", "merge")
+PHASES = ("init", "categorize", "expand", "symbolic", "merge", "predict", "compact")
 MTX_COMPAT_VALUES = 1
 EXPAND_KBINS = 0
 EXPAND_ROWS = 1
@@ -67,7 +67,13 @@ class CSR64(Structure):
 
 class Config(Structure):
     _fields_ = [("tile_log2", c_int32), ("item_products", c_int32), ("long_row_ub", c_int64),
-                ("xcd_swizzle", c_int32), ("timing", c_int32), ("expand_mode", c_int32)]
+                ("xcd_swizzle", c_int32), ("timing", c_int32), ("expand_mode", c_int32),
+                ("pred_margin", c_float), ("pred_slack", c_int32)]
+
+
+class LearnedStats(Structure):
+    _fields_ = [("vrows", c_int64), ("overflow_vrows", c_int64), ("unrecovered", c_int64),
+                ("padded_slots", c_int64), ("fallback_slots", c_int64), ("nnz", c_int64)]
 
 
 TRIPLET_DTYPE = np.dtype([("ridx", np.int32), ("cidx", np.int32), ("val", np.float32)])
@@ -128,6 +134,21 @@ _SIGS = {
     "spgGetIntermediateByRow": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_int32),
                                         POINTER(c_float)]),
     "spgGetPhaseTimes": (c_int, [c_void_p, POINTER(c_double)]),
+    "spgPredictorCreate": (c_int, [POINTER(c_void_p), c_int, c_uint64]),
+    "spgPredictorDestroy": (c_int, [c_void_p]),
+    "spgPredictorGetLastError": (c_char_p, [c_void_p]),
+    "spgPredictorNumParams": (c_int, []),
+    "spgPredictorAddSamples": (c_int, [c_void_p, c_void_p]),
+    "spgPredictorTrain": (c_int, [c_void_p, c_int, c_int, c_float, POINTER(c_float)]),
+    "spgPredictorGradient": (c_int, [c_void_p, POINTER(c_float), POINTER(c_float)]),
+    "spgPredictorGetWeights": (c_int, [c_void_p, POINTER(c_float), POINTER(c_float)]),
+    "spgPredictorSetWeights": (c_int, [c_void_p, POINTER(c_float), POINTER(c_float)]),
+    "spgPredictorSave": (c_int, [c_void_p, c_char_p]),
+    "spgPredictorLoad": (c_int, [c_void_p, c_char_p]),
+    "spgPredictRows": (c_int, [c_void_p, c_void_p, POINTER(c_float)]),
+    "spgSetRowPredictions": (c_int, [c_void_p, POINTER(c_float)]),
+    "spgLearnedMergeGEMM": (c_int, [c_void_p, POINTER(LearnedStats)]),
+    "spgMultiplyLearned": (c_int, [c_void_p, c_void_p, POINTER(LearnedStats)]),
 }
 
 
@@ -421,6 +442,93 @@ class Context:
         ms = np.zeros(len(PHASES), np.float64)
         self._c(lib().spgGetPhaseTimes(self.h, _ptr(ms, c_double)), "spgGetPhaseTimes")
         return dict(zip(PHASES, ms.tolist()))
+
+    # learned allocation
+    def predict_rows(self, predictor, m):
+        """Runs the MLP over the row features (after init_gemm); returns the
+        m predicted row counts and keeps them for learned_merge_gemm."""
+        pred = np.zeros(max(m, 1), np.float32)
+        self._c(lib().spgPredictRows(self.h, predictor.h, _ptr(pred, c_float)), "spgPredictRows")
+        return pred[:m]
+
+    def set_row_predictions(self, pred):
+        pred = np.ascontiguousarray(pred, np.float32)
+        self._c(lib().spgSetRowPredictions(self.h, _ptr(pred, c_float)), "spgSetRowPredictions")
+
+    def learned_merge_gemm(self):
+        st = LearnedStats()
+        self._c(lib().spgLearnedMergeGEMM(self.h, byref(st)), "spgLearnedMergeGEMM")
+        return {f: getattr(st, f) for f, _ in LearnedStats._fields_}
+
+    def multiply_learned(self, predictor):
+        st = LearnedStats()
+        self._c(lib().spgMultiplyLearned(self.h, predictor.h, byref(st)), "spgMultiplyLearned")
+        return {f: getattr(st, f) for f, _ in LearnedStats._fields_}
+
+
+class Predictor:
+    """The 4-64-64-1 row-count MLP (weights on the GPU)."""
+
+    def __init__(self, device=0, seed=0):
+        h = c_void_p()
+        _check(lib().spgPredictorCreate(byref(h), device, seed), what="spgPredictorCreate")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().spgPredictorDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, status, what):
+        if status != 0:
+            m = lib().spgPredictorGetLastError(self.h)
+            raise SpgError(status, f"{what}: {m.decode() if m else ''}")
+
+    @staticmethod
+    def num_params():
+        return int(lib().spgPredictorNumParams())
+
+    def add_samples(self, ctx):
+        self._c(lib().spgPredictorAddSamples(self.h, ctx.h), "spgPredictorAddSamples")
+
+    def train(self, epochs, batch=4096, lr=1e-3):
+        loss = c_float(0.0)
+        self._c(lib().spgPredictorTrain(self.h, epochs, batch, lr, byref(loss)), "spgPredictorTrain")
+        return loss.value
+
+    def gradient(self):
+        """(loss, grad) of the mean squared error over all samples."""
+        g = np.zeros(self.num_params(), np.float32)
+        loss = c_float(0.0)
+        self._c(lib().spgPredictorGradient(self.h, _ptr(g, c_float), byref(loss)),
+                "spgPredictorGradient")
+        return loss.value, g
+
+    def get_weights(self):
+        w = np.zeros(self.num_params(), np.float32)
+        norm = np.zeros(8, np.float32)
+        self._c(lib().spgPredictorGetWeights(self.h, _ptr(w, c_float), _ptr(norm, c_float)),
+                "spgPredictorGetWeights")
+        return w, norm
+
+    def set_weights(self, w, norm):
+        w = np.ascontiguousarray(w, np.float32)
+        norm = np.ascontiguousarray(norm, np.float32)
+        self._c(lib().spgPredictorSetWeights(self.h, _ptr(w, c_float), _ptr(norm, c_float)),
+                "spgPredictorSetWeights")
+
+    def save(self, path):
+        self._c(lib().spgPredictorSave(self.h, path.encode()), "spgPredictorSave")
+
+    def load(self, path):
+        self._c(lib().spgPredictorLoad(self.h, path.encode()), "spgPredictorLoad")
+
 
 
 def spgemm(A, B, device=0, ctx=None, simple=False):

@@ -146,6 +146,8 @@ int spgCreate(spg_context** out, int device) {
     c->cfg.xcd_swizzle = 1;
     c->cfg.timing = 1;
     c->cfg.expand_mode = SPG_EXPAND_KBINS;
+    c->cfg.pred_margin = 0.25f;
+    c->cfg.pred_slack = 8;
     int rc = SPG_OK;
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -178,13 +180,14 @@ int spgDestroy(spg_context* c) {
     operand_release(c->A);
     operand_release(c->B);
     DevBuf* bufs[] = {&c->ub, &c->ntile, &c->vstart, &c->vrow_row, &c->vlo, &c->vwid,
-                      &c->counters, &c->arow_ptr, &c->arow_k, &c->arow_v, &c->tilebits,
+                      &c->counters, &c->arow_ptr, &c->arow_k, &c->arow_v, &c->tilebits, &c->feat,
                       &c->bsplit,
                       &c->vub,
                       &c->vbase, &c->cursor, &c->kbin, &c->kperm, &c->kbin_off, &c->kitems,
                       &c->kitem_off, &c->icol, &c->ival, &c->vclass, &c->vlist,
                       &c->vclass_off, &c->vcnt, &c->vout, &c->crow, &c->ccol, &c->cval,
-                      &c->scratch};
+                      &c->pred, &c->vcap, &c->pbase, &c->fbase, &c->pcol, &c->pval,
+                      &c->fcol, &c->fval, &c->scratch};
     for (DevBuf* b : bufs) release(*b);
     if (c->pinned_small.p) (void)hipHostFree(c->pinned_small.p);
     for (auto& e : c->timer.ev)
@@ -219,6 +222,10 @@ int spgSetConfig(spg_context* c, const spg_config* cfg) {
         return fail(c, SPG_EINVAL, "long_row_ub must be in [1, 4096]");
     if (cfg->expand_mode != SPG_EXPAND_KBINS && cfg->expand_mode != SPG_EXPAND_ROWS)
         return fail(c, SPG_EINVAL, "expand_mode must be SPG_EXPAND_KBINS or SPG_EXPAND_ROWS");
+    if (!(cfg->pred_margin >= 0.f && cfg->pred_margin <= 100.f))
+        return fail(c, SPG_EINVAL, "pred_margin must be in [0, 100]");
+    if (cfg->pred_slack < 0 || cfg->pred_slack > (1 << 20))
+        return fail(c, SPG_EINVAL, "pred_slack must be in [0, 2^20]");
     c->cfg = *cfg;
     c->stage = std::min(c->stage, 0);
     return SPG_OK;
@@ -365,6 +372,7 @@ int spgInitGEMM(spg_context* c, cm* C, int64_t* nprod) {
         return fail(c, SPG_EINVAL, "A.num_cols != B.num_rows");
     SPG_HIP(hipSetDevice(c->device));
     c->stage = 0;
+    c->have_pred = false;
     c->m = c->A.num_rows;
     c->K = c->A.num_cols;
     c->n = c->B.num_cols;
@@ -381,6 +389,7 @@ int spgInitGEMM(spg_context* c, cm* C, int64_t* nprod) {
     SPG_TRY(ensure(c, c->arow_ptr, sizeof(int32_t) * ((size_t)c->m + 1)));
     SPG_TRY(ensure(c, c->arow_k, sizeof(int32_t) * ((size_t)c->A.nnz + 1)));
     SPG_TRY(ensure(c, c->arow_v, sizeof(float) * ((size_t)c->A.nnz + 1)));
+    SPG_TRY(ensure(c, c->feat, sizeof(float) * 4 * ((size_t)c->m + 1)));
     SPG_TRY(launch_init(c));
     int32_t V = 0;
     SPG_TRY(read_scalar<int32_t>(c, (const int32_t*)c->vstart.p + c->m, &V));
@@ -508,6 +517,57 @@ int spgMultiply(spg_context* c) {
     SPG_TRY(spgBinGEMM(c));
     SPG_TRY(spgSymbolicGEMM(c, nullptr));
     return spgMergeGEMM(c);
+}
+
+int spgPredictRows(spg_context* c, const spg_predictor* p, float* pred) {
+    if (!c || !p) return SPG_EINVAL;
+    if (c->stage < 1) return fail(c, SPG_EORDER, "InitGEMM must run first");
+    if (predictor_device(p) != c->device)
+        return fail(c, SPG_EINVAL, "predictor and context are on different devices");
+    SPG_HIP(hipSetDevice(c->device));
+    SPG_TRY(ensure(c, c->pred, sizeof(float) * ((size_t)c->m + 1)));
+    tmark(c, SPG_PHASE_PREDICT, 0);
+    SPG_TRY(predictor_run(c, p, 1.f, (float*)c->pred.p));
+    tmark(c, SPG_PHASE_PREDICT, 1);
+    c->have_pred = true;
+    if (pred) SPG_TRY(copy_d2h(c, pred, c->pred.p, sizeof(float) * (size_t)c->m));
+    return SPG_OK;
+}
+
+int spgSetRowPredictions(spg_context* c, const float* pred) {
+    if (!c || (!pred && c->m > 0)) return SPG_EINVAL;
+    if (c->stage < 1) return fail(c, SPG_EORDER, "InitGEMM must run first");
+    SPG_HIP(hipSetDevice(c->device));
+    SPG_TRY(ensure(c, c->pred, sizeof(float) * ((size_t)c->m + 1)));
+    if (c->m > 0) {
+        SPG_HIP(hipMemcpyAsync(c->pred.p, pred, sizeof(float) * (size_t)c->m,
+                               hipMemcpyHostToDevice, c->stream));
+        SPG_HIP(hipStreamSynchronize(c->stream));
+    }
+    c->have_pred = true;
+    return SPG_OK;
+}
+
+// Merge sized by the row predictions instead of the symbolic pass; rows
+// that overflow their predicted capacity are merged again exactly.
+int spgLearnedMergeGEMM(spg_context* c, spg_learned_stats* st) {
+    if (!c) return SPG_EINVAL;
+    if (c->stage < 3) return fail(c, SPG_EORDER, "BinGEMM or SimpleGEMM must run first");
+    if (!c->have_pred)
+        return fail(c, SPG_EORDER, "spgPredictRows or spgSetRowPredictions must run first");
+    SPG_HIP(hipSetDevice(c->device));
+    SPG_TRY(scratch_prepare(c, c->V + 1));
+    SPG_TRY(launch_learned_merge(c, st));
+    c->stage = 5;
+    return SPG_OK;
+}
+
+int spgMultiplyLearned(spg_context* c, const spg_predictor* p, spg_learned_stats* st) {
+    SPG_TRY(spgInitGEMM(c, nullptr, nullptr));
+    SPG_TRY(spgCategorizeGEMM(c, nullptr));
+    SPG_TRY(spgBinGEMM(c));
+    SPG_TRY(spgPredictRows(c, p, nullptr));
+    return spgLearnedMergeGEMM(c, st);
 }
 
 int spgGetC(spg_context* c, spg_csr64* out) {

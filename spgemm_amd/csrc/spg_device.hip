@@ -339,25 +339,54 @@ __device__ __forceinline__ T wave_sum(T x) {
     return x;
 }
 
-// One wave per row: ub_i = sum of nnz(B row k) over the row's k; tiles.
+template <typename T>
+__device__ __forceinline__ T wave_max(T x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const T y = __shfl_xor(x, o, 64);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+
+// One wave per row: ub_i = sum of nnz(B row k) over the row's k (the flop
+// upper bound), the column tiling of the row, and the predictor's row
+// features: log1p of nnz(A row i), ub_i, max_k nnz(B row k) and the
+// standard deviation of the nnz(B row k).
 __global__ __launch_bounds__(kBlock) void k_row_bound(const int* __restrict__ rptr,
                                                        const int* __restrict__ rk,
                                                        const int* __restrict__ b_ptr, int m,
                                                        long long long_ub, int T,
                                                        long long* __restrict__ ub,
-                                                       int* __restrict__ ntile) {
+                                                       int* __restrict__ ntile,
+                                                       float4* __restrict__ feat) {
     const long long i = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
     if (i >= m) return;
-    long long s = 0;
-    const int e1 = rptr[i + 1];
-    for (int e = rptr[i] + lane_id(); e < e1; e += 64) {
+    long long s = 0, s2 = 0;
+    int mx = 0;
+    const int e0 = rptr[i], e1 = rptr[i + 1];
+    for (int e = e0 + lane_id(); e < e1; e += 64) {
         const int k = rk[e];
-        s += b_ptr[k + 1] - b_ptr[k];
+        const long long b = b_ptr[k + 1] - b_ptr[k];
+        s += b;
+        s2 += b * b;
+        mx = max(mx, (int)b);
     }
     s = wave_sum(s);
+    s2 = wave_sum(s2);
+    mx = wave_max(mx);
     if (lane_id() == 0) {
         ub[i] = s;
         ntile[i] = (T > 1 && s > long_ub) ? T : 1;
+        const int r = e1 - e0;
+        double var = 0.0;
+        if (r > 0) {
+            const double mean = (double)s / r;
+            var = (double)s2 / r - mean * mean;
+            if (var < 0.0) var = 0.0;
+        }
+        feat[i] = make_float4(log1pf((float)r), log1pf((float)s), log1pf((float)mx),
+                              log1pf((float)sqrt(var)));
     }
 }
 
@@ -482,7 +511,7 @@ int launch_init(spg_context* c) {
         hipLaunchKernelGGL(k_row_bound, dim3((unsigned)nbm), dim3(kBlock), 0, c->stream,
                            (const int*)rptr, (const int*)rk, c->B.ptr, m,
                            (long long)c->cfg.long_row_ub, c->T, (long long*)c->ub.p,
-                           (int*)c->ntile.p);
+                           (int*)c->ntile.p, (float4*)c->feat.p);
         SPG_HIP(hipGetLastError());
     }
     return scan_i32(c, (const int32_t*)c->ntile.p, (int32_t*)c->vstart.p, m);
@@ -1109,24 +1138,24 @@ __device__ __forceinline__ unsigned hash_slot(int key, int log2h) {
 }
 
 // Insert key into an LDS open-addressing table; returns the slot and sets
-// fresh when the key was not present.  The table is never more than half
-// full, so the probe sequence ends.
+// fresh when the key was not present, or returns -1 when all H slots hold
+// other keys (possible only when the table was sized from a prediction; a
+// table sized from an exact count or an upper bound is at most half full).
 template <int LOG2H>
 __device__ __forceinline__ int hash_insert(int* keys, int key, bool& fresh) {
     constexpr int H = 1 << LOG2H;
     unsigned h = hash_slot(key, LOG2H);
-    for (;;) {
+    fresh = false;
+    for (int probe = 0; probe < H; ++probe) {
         int old = atomicCAS(&keys[h], -1, key);
         if (old == -1) {
             fresh = true;
             return (int)h;
         }
-        if (old == key) {
-            fresh = false;
-            return (int)h;
-        }
+        if (old == key) return (int)h;
         h = (h + 1) & (H - 1);
     }
+    return -1;
 }
 
 // --------------------------------------------------------------- symbolic
@@ -1391,41 +1420,59 @@ __global__ void k_vclass_merge(long long V, const long long* __restrict__ vcnt,
     cls[v] = lg - kHashMinLog2;
 }
 
+// cnt_out (may be null) receives the row's distinct-column count, or -1
+// when the table (sized from a predicted count) could not hold the row; the
+// row's output is then incomplete and the caller merges it again.
 template <int LOG2H>
 __global__ __launch_bounds__(kBlock) void k_merge_hash(
     const int* __restrict__ vlist, const long long* __restrict__ vbase,
     const int* __restrict__ icol, const float* __restrict__ ival,
-    const long long* __restrict__ vout, int* __restrict__ ccol, float* __restrict__ cval) {
+    const long long* __restrict__ vout, int* __restrict__ ccol, float* __restrict__ cval,
+    long long* __restrict__ cnt_out) {
     constexpr int H = 1 << LOG2H;
-    constexpr int P = H / 2;  // at most H/2 distinct keys
+    constexpr int P = H / 2;  // at most H/2 distinct keys are sorted
     __shared__ int keys[H];
     __shared__ double vals[H];
     __shared__ int skey[P];
     __shared__ double sval[P];
     __shared__ int cnt;
+    __shared__ int full;
     const int v = vlist[blockIdx.x];
     for (int h = threadIdx.x; h < H; h += kBlock) {
         keys[h] = -1;
         vals[h] = 0.0;
     }
-    if (threadIdx.x == 0) cnt = 0;
+    if (threadIdx.x == 0) {
+        cnt = 0;
+        full = 0;
+    }
     __syncthreads();
     const long long s1 = vbase[v + 1];
     for (long long s = vbase[v] + threadIdx.x; s < s1; s += kBlock) {
         bool fresh;
         const int h = hash_insert<LOG2H>(keys, icol[s], fresh);
-        unsafeAtomicAdd(&vals[h], (double)ival[s]);
+        if (h >= 0)
+            unsafeAtomicAdd(&vals[h], (double)ival[s]);
+        else
+            full = 1;
     }
     __syncthreads();
     for (int h = threadIdx.x; h < H; h += kBlock) {
         if (keys[h] != -1) {
             const int q = atomicAdd(&cnt, 1);
-            skey[q] = keys[h];
-            sval[q] = vals[h];
+            if (q < P) {
+                skey[q] = keys[h];
+                sval[q] = vals[h];
+            }
         }
     }
     __syncthreads();
+    if (full || cnt > P) {  // only with a predicted table size
+        if (threadIdx.x == 0 && cnt_out) cnt_out[v] = -1;
+        return;
+    }
     const int c = cnt;
+    if (threadIdx.x == 0 && cnt_out) cnt_out[v] = c;
     int np = 1;
     while (np < c) np <<= 1;
     for (int q = c + threadIdx.x; q < np; q += kBlock) skey[q] = 0x7fffffff;
@@ -1492,7 +1539,7 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
     const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
     const int* __restrict__ vlo, const int* __restrict__ vwid, const int* __restrict__ icol,
     const float* __restrict__ ival, const long long* __restrict__ vout, int* __restrict__ ccol,
-    float* __restrict__ cval) {
+    float* __restrict__ cval, long long* __restrict__ cnt_out) {
     constexpr int WMAX = 1 << kTileMaxLog2;
     constexpr int NW = WMAX / 32;
     static_assert(NW <= kPT, "one bitmap word per thread");
@@ -1560,6 +1607,7 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
         unsigned b = threadIdx.x < nw ? bits[threadIdx.x] : 0u;
         int tot;
         const int pre = block_exscan_n<int, kPT>(__popc(b), &tot, sh);
+        if (cnt_out && threadIdx.x == 0) cnt_out[v] = tot;
         long long o = o0 + pre;
         while (b) {
             const int bit = __ffs(b) - 1;
@@ -1589,16 +1637,19 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
 
 // Tile-width virtual rows with at most CAP columns: one wave per row, no
 // workgroup barriers.  The row's column bitmap (stored by the symbolic
-// pass) and a prefix popcount per word give every column its rank, i.e.
-// its output position; products are accumulated in FP64 at acc[rank], the
-// column is recorded at col[rank], and the row is written out with fully
-// coalesced stores.
-template <int CAP, int WPB>
+// pass, or built here from the products when BUILD) and a prefix popcount
+// per word give every column its rank, i.e. its output position; products
+// are accumulated in FP64 at acc[rank], the column is recorded at
+// col[rank], and the row is written out with fully coalesced stores.  With
+// BUILD, cnt_out receives the row's distinct-column count (which may exceed
+// its output capacity; only the capacity is written).
+template <int CAP, int WPB, bool BUILD>
 __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
     const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
     const int* __restrict__ vlo, const unsigned* __restrict__ tilebits,
     const int* __restrict__ icol, const float* __restrict__ ival,
-    const long long* __restrict__ vout, int* __restrict__ ccol, float* __restrict__ cval) {
+    const long long* __restrict__ vout, int* __restrict__ ccol, float* __restrict__ cval,
+    long long* __restrict__ cnt_out) {
     __shared__ double accall[WPB][CAP];
     __shared__ int colall[WPB][CAP];
     __shared__ unsigned bitsall[WPB][kTileWords];
@@ -1613,9 +1664,31 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
         const int v = vlist[j];
         const long long s0 = vbase[v], s1 = vbase[v + 1];
         const long long o0 = vout[v];
-        const int cnt = (int)min((long long)CAP, vout[v + 1] - o0);
+        int cnt = (int)min((long long)CAP, vout[v + 1] - o0);
         const int lo = vlo[v];
-        const uint4 wq = *(const uint4*)(tilebits + (long long)v * kTileWords + lane * 4);
+        uint4 wq;
+        if constexpr (BUILD) {
+            *(uint4*)(bits + lane * 4) = make_uint4(0u, 0u, 0u, 0u);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (long long sb = s0 & ~3LL; sb < s1; sb += 4 * 64) {
+                const long long base = sb + (long long)lane * 4;
+                const int4 q = base < s1 ? *(const int4*)(icol + base) : make_int4(0, 0, 0, 0);
+                const int cc[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    if (base + e >= s0 && base + e < s1) {
+                        const int c = cc[e] - lo;
+                        atomicOr(&bits[c >> 5], 1u << (c & 31));
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            wq = *(const uint4*)(bits + lane * 4);
+        } else {
+            wq = *(const uint4*)(tilebits + (long long)v * kTileWords + lane * 4);
+        }
         const int p0 = __popc(wq.x), p1 = __popc(wq.y), p2 = __popc(wq.z), p3 = __popc(wq.w);
         const int mine = p0 + p1 + p2 + p3;
         int inc = mine;
@@ -1625,7 +1698,13 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
             if (lane >= o) inc += y;
         }
         const int ex = inc - mine;
-        *(uint4*)(bits + lane * 4) = wq;
+        if constexpr (BUILD) {
+            const int total = __shfl(inc, 63, 64);
+            if (lane == 0) cnt_out[v] = total;
+            cnt = min(cnt, total);  // the capacity may exceed the count
+        } else {
+            *(uint4*)(bits + lane * 4) = wq;
+        }
         pre[lane * 4 + 0] = (unsigned short)ex;
         pre[lane * 4 + 1] = (unsigned short)(ex + p0);
         pre[lane * 4 + 2] = (unsigned short)(ex + p0 + p1);
@@ -1675,7 +1754,7 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
     const int* __restrict__ vlist, long long nlist, unsigned long long* __restrict__ ctr,
     const long long* __restrict__ vbase, int n, const int* __restrict__ icol,
     const float* __restrict__ ival, const long long* __restrict__ vout,
-    int* __restrict__ ccol, float* __restrict__ cval) {
+    int* __restrict__ ccol, float* __restrict__ cval, long long* __restrict__ cnt_out) {
     constexpr int WPT = kRankMaxWords / kPT;  // 16 words per thread
     __shared__ unsigned bits[kRankMaxWords];
     __shared__ unsigned short pre[kRankMaxWords / 2];  // rank before each word pair
@@ -1710,6 +1789,7 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
         for (int q = 0; q < WPT; ++q) mine += (w0 + q < nw) ? __popc(bits[w0 + q]) : 0;
         int tot;
         int run = block_exscan_n<int, kPT>(mine, &tot, sh);
+        if (cnt_out && threadIdx.x == 0) cnt_out[v] = tot;
 #pragma unroll
         for (int q = 0; q < WPT; q += 2) {
             pre[(w0 + q) >> 1] = (unsigned short)run;
@@ -1760,12 +1840,73 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
     }
 }
 
+// Where a merge writes: virtual row v's columns go to ocol/oval from
+// obase[v] on, at most obase[v+1] - obase[v] of them; cnt (may be null)
+// receives each row's distinct-column count.
+struct MergeOut {
+    const long long* obase;
+    int* ocol;
+    float* oval;
+    long long* cnt;
+};
+
 template <int LOG2H>
-static void launch_merge_hash(spg_context* c, const int* list, long long cnt) {
+static void launch_merge_hash(spg_context* c, const int* list, long long cnt, const MergeOut& o) {
     hipLaunchKernelGGL(k_merge_hash<LOG2H>, dim3((unsigned)cnt), dim3(kBlock), 0, c->stream,
                        list, (const long long*)c->vbase.p, (const int*)c->icol.p,
-                       (const float*)c->ival.p, (const long long*)c->vout.p, (int*)c->ccol.p,
-                       (float*)c->cval.p);
+                       (const float*)c->ival.p, o.obase, o.ocol, o.oval, o.cnt);
+}
+
+// One merge kernel over the virtual rows list[0..cnt) of class b.  build:
+// tile rows build their column bitmap themselves (no symbolic pass ran).
+static int launch_merge_class(spg_context* c, int b, const int* list, long long cnt,
+                              const MergeOut& o, bool build) {
+    unsigned long long* ctr = (unsigned long long*)c->counters.p;
+    const long long* vbase = (const long long*)c->vbase.p;
+    const int* vlo = (const int*)c->vlo.p;
+    const int* icol = (const int*)c->icol.p;
+    const float* ival = (const float*)c->ival.p;
+    const unsigned* tb = (const unsigned*)c->tilebits.p;
+    if (b == kClassTileS) {
+        const dim3 g(persistent_grid(c, (cnt + 7) / 8, 4));
+        if (build)
+            hipLaunchKernelGGL((k_merge_tile_wave<256, 8, true>), g, dim3(512), 0, c->stream, list,
+                               cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval, o.cnt);
+        else
+            hipLaunchKernelGGL((k_merge_tile_wave<256, 8, false>), g, dim3(512), 0, c->stream, list,
+                               cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval, o.cnt);
+    } else if (b == kClassTileM) {
+        const dim3 g(persistent_grid(c, (cnt + 3) / 4, 2));
+        if (build)
+            hipLaunchKernelGGL((k_merge_tile_wave<1024, 4, true>), g, dim3(256), 0, c->stream, list,
+                               cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval, o.cnt);
+        else
+            hipLaunchKernelGGL((k_merge_tile_wave<1024, 4, false>), g, dim3(256), 0, c->stream,
+                               list, cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval,
+                               o.cnt);
+    } else if (b == kClassTile) {
+        hipLaunchKernelGGL(k_merge_tile, dim3(persistent_grid(c, cnt, 2)), dim3(kPT), 0,
+                           c->stream, list, cnt, vbase, vlo, (const int*)c->vwid.p, icol, ival,
+                           o.obase, o.ocol, o.oval, o.cnt);
+    } else if (b == kClassRank) {
+        hipLaunchKernelGGL(k_merge_rank, dim3(persistent_grid(c, cnt, 2)), dim3(kPT), 0,
+                           c->stream, list, cnt, ctr + 2, vbase, c->n, icol, ival, o.obase, o.ocol,
+                           o.oval, o.cnt);
+    } else {
+        switch (b + kHashMinLog2) {
+            case 6: launch_merge_hash<6>(c, list, cnt, o); break;
+            case 7: launch_merge_hash<7>(c, list, cnt, o); break;
+            case 8: launch_merge_hash<8>(c, list, cnt, o); break;
+            case 9: launch_merge_hash<9>(c, list, cnt, o); break;
+            case 10: launch_merge_hash<10>(c, list, cnt, o); break;
+            case 11: launch_merge_hash<11>(c, list, cnt, o); break;
+            case 12: launch_merge_hash<12>(c, list, cnt, o); break;
+            case 13: launch_merge_hash<13>(c, list, cnt, o); break;
+            default: return fail(c, SPG_EINVAL, "bad merge class");
+        }
+    }
+    SPG_HIP(hipGetLastError());
+    return SPG_OK;
 }
 
 int launch_merge(spg_context* c) {
@@ -1782,53 +1923,263 @@ int launch_merge(spg_context* c) {
     long long off[kNumVClasses + 1];
     SPG_TRY(class_offsets(c, off));
     const int* vl = (const int*)c->vlist.p;
-    unsigned long long* ctr = (unsigned long long*)c->counters.p;
+    const MergeOut o{(const long long*)c->vout.p, (int*)c->ccol.p, (float*)c->cval.p, nullptr};
     tmark(c, SPG_PHASE_MERGE, 0);
     for (int b = kNumVClasses - 1; b >= 0; --b) {
         const long long cnt = off[b + 1] - off[b];
-        if (cnt <= 0) continue;
-        const int* list = vl + off[b];
-        if (b == kClassTileS) {
-            hipLaunchKernelGGL((k_merge_tile_wave<256, 8>),
-                               dim3(persistent_grid(c, (cnt + 7) / 8, 4)), dim3(512), 0, c->stream,
-                               list, cnt, (const long long*)c->vbase.p, (const int*)c->vlo.p,
-                               (const unsigned*)c->tilebits.p, (const int*)c->icol.p,
-                               (const float*)c->ival.p, (const long long*)c->vout.p,
-                               (int*)c->ccol.p, (float*)c->cval.p);
-        } else if (b == kClassTileM) {
-            hipLaunchKernelGGL((k_merge_tile_wave<1024, 4>),
-                               dim3(persistent_grid(c, (cnt + 3) / 4, 2)), dim3(256), 0, c->stream,
-                               list, cnt, (const long long*)c->vbase.p, (const int*)c->vlo.p,
-                               (const unsigned*)c->tilebits.p, (const int*)c->icol.p,
-                               (const float*)c->ival.p, (const long long*)c->vout.p,
-                               (int*)c->ccol.p, (float*)c->cval.p);
-        } else if (b == kClassTile) {
-            hipLaunchKernelGGL(k_merge_tile, dim3(persistent_grid(c, cnt, 2)), dim3(kPT), 0,
-                               c->stream, list, cnt, (const long long*)c->vbase.p,
-                               (const int*)c->vlo.p, (const int*)c->vwid.p,
-                               (const int*)c->icol.p, (const float*)c->ival.p,
-                               (const long long*)c->vout.p, (int*)c->ccol.p, (float*)c->cval.p);
-        } else if (b == kClassRank) {
-            hipLaunchKernelGGL(k_merge_rank, dim3(persistent_grid(c, cnt, 2)), dim3(kPT), 0,
-                               c->stream, list, cnt, ctr + 2, (const long long*)c->vbase.p, c->n,
-                               (const int*)c->icol.p, (const float*)c->ival.p,
-                               (const long long*)c->vout.p, (int*)c->ccol.p, (float*)c->cval.p);
-        } else {
-            switch (b + kHashMinLog2) {
-                case 6: launch_merge_hash<6>(c, list, cnt); break;
-                case 7: launch_merge_hash<7>(c, list, cnt); break;
-                case 8: launch_merge_hash<8>(c, list, cnt); break;
-                case 9: launch_merge_hash<9>(c, list, cnt); break;
-                case 10: launch_merge_hash<10>(c, list, cnt); break;
-                case 11: launch_merge_hash<11>(c, list, cnt); break;
-                case 12: launch_merge_hash<12>(c, list, cnt); break;
-                case 13: launch_merge_hash<13>(c, list, cnt); break;
-                default: return fail(c, SPG_EINVAL, "bad merge class");
-            }
-        }
-        SPG_HIP(hipGetLastError());
+        if (cnt > 0) SPG_TRY(launch_merge_class(c, b, vl + off[b], cnt, o, false));
     }
     tmark(c, SPG_PHASE_MERGE, 1);
+    return SPG_OK;
+}
+
+// ---------------------------------------------------------- learned merge
+// The symbolic pass is replaced by a predicted count per C row: every
+// virtual row gets an output capacity from the prediction, is merged into a
+// padded buffer by the kernel of the class that capacity selects (tile rows
+// build their column bitmap themselves), and reports its true distinct
+// count.  Rows whose count exceeds the capacity (or whose predicted-size
+// hash table filled up) are merged again with their upper bound
+// min(products, width) as capacity, which always fits; the exact counts
+// then give C's row offsets and the rows are compacted into CSR.
+
+// Adds the number of lanes with flag set to *ctr (one atomic per wave).
+__device__ __forceinline__ void wave_count_add(unsigned long long* ctr, bool flag) {
+    const unsigned long long mask = __ballot(flag);
+    const unsigned long long act = __ballot(1);
+    if (mask && lane_id() == __ffsll((long long)act) - 1)
+        atomicAdd(ctr, (unsigned long long)__popcll(mask));
+}
+
+__device__ __forceinline__ int hash_class_for(long long cap) {
+    int lg = kHashMinLog2;
+    while ((1LL << lg) < 2 * cap) ++lg;
+    return lg - kHashMinLog2;
+}
+
+// Capacity and class of every virtual row.  A tile of a long row gets the
+// share of its row's prediction proportional to its products.
+__global__ void k_learned_class(long long V, const int* __restrict__ vrow_row,
+                                const long long* __restrict__ vub, const int* __restrict__ vwid,
+                                const long long* __restrict__ ub, const float* __restrict__ pred,
+                                int W, int n, float margin, int slack,
+                                long long* __restrict__ vcap, int* __restrict__ cls,
+                                unsigned long long* __restrict__ nlive) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool live = false;
+    if (v < V) {
+        const long long u = vub[v];
+        long long cap = 0;
+        int k = -1;
+        if (u > 0) {
+            live = true;
+            const int width = vwid[v];
+            const long long bound = min(u, (long long)width);
+            const int i = vrow_row[v];
+            double share = (double)pred[i] * ((double)u / (double)ub[i]);
+            if (!(share > 0.0)) share = 0.0;  // negative or NaN
+            const double want = ceil(share * (1.0 + (double)margin)) + (double)slack;
+            cap = want < 1.0 ? 1 : (want >= (double)bound ? bound : (long long)want);
+            if (width <= W)
+                k = cap <= 256 ? kClassTileS : (cap <= 1024 ? kClassTileM : kClassTile);
+            else if (cap > kRankMinCount && n <= kRankMaxWords * 32)
+                k = kClassRank;
+            else
+                k = hash_class_for(cap);  // cap <= u <= 4096: at most 8192 slots
+        }
+        vcap[v] = cap;
+        cls[v] = k;
+    }
+    wave_count_add(nlive, live);
+}
+
+// Rows that did not fit their capacity (count > capacity, or -1 from a full
+// hash table) get the upper bound as fallback capacity (fcap) and a class
+// that merges any count up to it; all other rows get 0 / -1.
+__global__ void k_learned_check(long long V, const long long* __restrict__ vcnt,
+                                const long long* __restrict__ pbase,
+                                const long long* __restrict__ vub, const int* __restrict__ vwid,
+                                int W, int n, long long* __restrict__ fcap, int* __restrict__ cls,
+                                unsigned long long* __restrict__ nover) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool over = false;
+    if (v < V) {
+        const long long cap = pbase[v + 1] - pbase[v], cnt = vcnt[v];
+        over = cap > 0 && (cnt < 0 || cnt > cap);
+        long long f = 0;
+        int k = -1;
+        if (over) {
+            const int width = vwid[v];
+            f = min(vub[v], (long long)width);
+            if (width <= W)
+                k = kClassTile;
+            else if (n <= kRankMaxWords * 32)
+                k = kClassRank;
+            else
+                k = hash_class_for(f);
+        }
+        fcap[v] = f;
+        cls[v] = k;
+    }
+    wave_count_add(nover, over);
+}
+
+// Moves every virtual row's columns and values from the padded (or the
+// fallback) buffer to its CSR position; one wave per row.
+__global__ __launch_bounds__(kPT) void k_learned_compact(
+    long long V, const long long* __restrict__ vcnt, const long long* __restrict__ vout,
+    const long long* __restrict__ pbase, const long long* __restrict__ fbase,
+    const int* __restrict__ pcol, const float* __restrict__ pval, const int* __restrict__ fcol,
+    const float* __restrict__ fval, int* __restrict__ ccol, float* __restrict__ cval,
+    unsigned long long* __restrict__ unrecovered) {
+    const int wv = threadIdx.x >> 6, lane = lane_id();
+    const long long stride = (long long)gridDim.x * (kPT / 64);
+    for (long long v = (long long)blockIdx.x * (kPT / 64) + wv; v < V; v += stride) {
+        long long cnt = vcnt[v];
+        if (cnt <= 0) continue;
+        const long long fb = fbase[v], fcap = fbase[v + 1] - fb;
+        const int* sc;
+        const float* sv;
+        long long lim;
+        if (fcap > 0) {
+            sc = fcol + fb;
+            sv = fval + fb;
+            lim = fcap;
+        } else {
+            const long long pb = pbase[v];
+            sc = pcol + pb;
+            sv = pval + pb;
+            lim = pbase[v + 1] - pb;
+        }
+        if (cnt > lim) {  // cannot happen: the fallback capacity is an upper bound
+            if (lane == 0) atomicAdd(unrecovered, 1ull);
+            cnt = lim;
+        }
+        const long long o = vout[v];
+        for (long long q = lane; q < cnt; q += 64) {
+            ccol[o + q] = sc[q];
+            cval[o + q] = sv[q];
+        }
+    }
+}
+
+// Counter slots of the learned merge in c->counters.
+constexpr int kCtrRank = 2, kCtrLive = 8, kCtrOver = 9, kCtrUnrec = 10;
+
+static int readback2(spg_context* c, const void* a, const void* b, long long* out) {
+    long long* h = (long long*)c->pinned_small.p;
+    SPG_HIP(hipMemcpyAsync(h, a, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+    if (b) SPG_HIP(hipMemcpyAsync(h + 1, b, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+    SPG_HIP(hipStreamSynchronize(c->stream));
+    out[0] = h[0];
+    out[1] = b ? h[1] : 0;
+    return SPG_OK;
+}
+
+int launch_learned_merge(spg_context* c, spg_learned_stats* st) {
+    const long long V = c->V;
+    const size_t V1 = (size_t)V + 1;
+    unsigned long long* ctr = (unsigned long long*)c->counters.p;
+    SPG_TRY(ensure(c, c->vcap, sizeof(long long) * V1));
+    SPG_TRY(ensure(c, c->pbase, sizeof(long long) * (V1 + 1)));
+    SPG_TRY(ensure(c, c->fbase, sizeof(long long) * (V1 + 1)));
+    SPG_TRY(ensure(c, c->vcnt, sizeof(long long) * V1));
+    SPG_TRY(ensure(c, c->vout, sizeof(long long) * (V1 + 1)));
+    SPG_TRY(ensure(c, c->crow, sizeof(long long) * ((size_t)c->m + 1)));
+    long long* vcap = (long long*)c->vcap.p;
+    long long* pbase = (long long*)c->pbase.p;
+    long long* fbase = (long long*)c->fbase.p;
+    long long* vcnt = (long long*)c->vcnt.p;
+    const unsigned gV = (unsigned)((V + kBlock - 1) / kBlock);
+
+    tmark(c, SPG_PHASE_MERGE, 0);
+    SPG_TRY(fill<unsigned long long>(c, ctr, 0ull, 64));
+    SPG_TRY(fill<long long>(c, vcnt, 0, V));
+    if (V > 0) {
+        hipLaunchKernelGGL(k_learned_class, dim3(gV), dim3(kBlock), 0, c->stream, V,
+                           (const int*)c->vrow_row.p, (const long long*)c->vub.p,
+                           (const int*)c->vwid.p, (const long long*)c->ub.p,
+                           (const float*)c->pred.p, c->W, c->n, c->cfg.pred_margin,
+                           c->cfg.pred_slack, vcap, (int*)c->vclass.p, ctr + kCtrLive);
+        SPG_HIP(hipGetLastError());
+    }
+    scratch_reset(c);
+    SPG_TRY(scan_i64(c, (const int64_t*)vcap, (int64_t*)pbase, V));
+    long long rb[2];
+    SPG_TRY(readback2(c, pbase + V, nullptr, rb));
+    const long long padded = rb[0];
+    SPG_TRY(ensure(c, c->pcol, sizeof(int) * (size_t)(padded + 16)));
+    SPG_TRY(ensure(c, c->pval, sizeof(float) * (size_t)(padded + 16)));
+    scratch_reset(c);
+    SPG_TRY(stable_partition(c, (const int32_t*)c->vclass.p, V, kNumVClasses,
+                             (int32_t*)c->vlist.p, (int64_t*)c->vclass_off.p));
+    long long off[kNumVClasses + 1];
+    SPG_TRY(class_offsets(c, off));
+    const int* vl = (const int*)c->vlist.p;
+    {
+        const MergeOut o{pbase, (int*)c->pcol.p, (float*)c->pval.p, vcnt};
+        for (int b = kNumVClasses - 1; b >= 0; --b) {
+            const long long cnt = off[b + 1] - off[b];
+            if (cnt > 0) SPG_TRY(launch_merge_class(c, b, vl + off[b], cnt, o, true));
+        }
+    }
+    // exact fallback for the rows that did not fit
+    if (V > 0) {
+        hipLaunchKernelGGL(k_learned_check, dim3(gV), dim3(kBlock), 0, c->stream, V,
+                           (const long long*)vcnt, (const long long*)pbase,
+                           (const long long*)c->vub.p, (const int*)c->vwid.p, c->W, c->n, vcap,
+                           (int*)c->vclass.p, ctr + kCtrOver);
+        SPG_HIP(hipGetLastError());
+    }
+    scratch_reset(c);
+    SPG_TRY(scan_i64(c, (const int64_t*)vcap, (int64_t*)fbase, V));
+    SPG_TRY(readback2(c, fbase + V, ctr + kCtrOver, rb));
+    const long long fslots = rb[0], nover = rb[1];
+    if (nover > 0) {
+        SPG_TRY(ensure(c, c->fcol, sizeof(int) * (size_t)(fslots + 16)));
+        SPG_TRY(ensure(c, c->fval, sizeof(float) * (size_t)(fslots + 16)));
+        scratch_reset(c);
+        SPG_TRY(stable_partition(c, (const int32_t*)c->vclass.p, V, kNumVClasses,
+                                 (int32_t*)c->vlist.p, (int64_t*)c->vclass_off.p));
+        SPG_TRY(class_offsets(c, off));
+        SPG_TRY(fill<unsigned long long>(c, ctr + kCtrRank, 0ull, 1));
+        const MergeOut o{fbase, (int*)c->fcol.p, (float*)c->fval.p, vcnt};
+        for (int b = kNumVClasses - 1; b >= 0; --b) {
+            const long long cnt = off[b + 1] - off[b];
+            if (cnt > 0) SPG_TRY(launch_merge_class(c, b, vl + off[b], cnt, o, true));
+        }
+    }
+    tmark(c, SPG_PHASE_MERGE, 1);
+
+    tmark(c, SPG_PHASE_COMPACT, 0);
+    scratch_reset(c);
+    SPG_TRY(scan_i64(c, (const int64_t*)vcnt, (int64_t*)c->vout.p, V));
+    SPG_TRY(readback2(c, (const long long*)c->vout.p + V, nullptr, rb));
+    c->nnzC = rb[0];
+    SPG_TRY(ensure(c, c->ccol, sizeof(int) * (size_t)(c->nnzC + 1)));
+    SPG_TRY(ensure(c, c->cval, sizeof(float) * (size_t)(c->nnzC + 1)));
+    SPG_TRY(launch_rowptr(c));
+    if (V > 0) {
+        const int* fcol = nover > 0 ? (const int*)c->fcol.p : nullptr;
+        const float* fval = nover > 0 ? (const float*)c->fval.p : nullptr;
+        hipLaunchKernelGGL(k_learned_compact, dim3(persistent_grid(c, (V + 7) / 8, 4)), dim3(kPT),
+                           0, c->stream, V, (const long long*)vcnt, (const long long*)c->vout.p,
+                           (const long long*)pbase, (const long long*)fbase,
+                           (const int*)c->pcol.p, (const float*)c->pval.p, fcol, fval,
+                           (int*)c->ccol.p, (float*)c->cval.p, ctr + kCtrUnrec);
+        SPG_HIP(hipGetLastError());
+    }
+    tmark(c, SPG_PHASE_COMPACT, 1);
+    if (st) {
+        long long live_unrec[2];
+        SPG_TRY(readback2(c, ctr + kCtrLive, ctr + kCtrUnrec, live_unrec));
+        st->vrows = live_unrec[0];
+        st->overflow_vrows = nover;
+        st->unrecovered = live_unrec[1];
+        st->padded_slots = padded;
+        st->fallback_slots = fslots;
+        st->nnz = c->nnzC;
+    }
     return SPG_OK;
 }
 

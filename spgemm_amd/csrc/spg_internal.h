@@ -71,6 +71,7 @@ struct spg_context {
     spg::DevBuf counters;  // work counters of persistent kernels
     spg::DevBuf arow_ptr, arow_k, arow_v;  // A transposed: row i -> its (k, a_ik)
     spg::DevBuf tilebits;          // column bitmap of every tile-width virtual row
+    spg::DevBuf feat;              // predictor features of every row (float4)
     int num_cu = 256;
     // categorize products
     spg::DevBuf kbin, kperm, kbin_off, kitems, kitem_off;
@@ -80,6 +81,10 @@ struct spg_context {
     spg::DevBuf icol, ival;
     // symbolic / merge
     spg::DevBuf vclass, vlist, vclass_off, vcnt, vout, crow, ccol, cval;
+    // learned merge: row predictions, per-virtual-row capacities, padded
+    // output (pbase/pcol/pval) and the fallback output (fbase/fcol/fval)
+    spg::DevBuf pred, vcap, pbase, fbase, pcol, pval, fcol, fval;
+    bool have_pred = false;
     // scratch for scans and partitions
     spg::DevBuf scratch;
     size_t scratch_top = 0;
@@ -121,6 +126,11 @@ int launch_expand_simple(spg_context* c);
 int launch_expand_rows(spg_context* c);
 int launch_symbolic(spg_context* c);
 int launch_merge(spg_context* c);
+int launch_learned_merge(spg_context* c, spg_learned_stats* st);
+int launch_rowptr(spg_context* c);
+// Runs the MLP over c's row features into d_pred (spg_predict.hip).
+int predictor_run(spg_context* c, const spg_predictor* p, float scale, float* d_pred);
+int predictor_device(const spg_predictor* p);
 int launch_regroup_intermediate(spg_context* c, int64_t* d_rowptr);
 
 }  // namespace spg

@@ -185,3 +185,67 @@ def test_faults_are_caught(orc, fault):
         orc.set_fault(orc.FAULT_NONE)
     assert any(caught)
     assert all(check_against_truth(orc, nm) for nm in ("tiny", "rect", "powerlaw"))
+
+
+# ---------------------------------------------------------------- predictor
+def _mlp_setup(seed, N=40):
+    from oracle import mlp
+    rng = np.random.default_rng(seed)
+    w = rng.normal(0, 0.3, mlp.NP)
+    x = rng.normal(0, 1, (N, 4)) * [1, 2, 1.5, 0.5] + [1, 3, 2, 1]
+    t = -np.abs(rng.normal(0, 1, N))
+    return mlp, w, mlp.normalization(x), x, t
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_mlp_gradient_matches_finite_differences(seed):
+    """Pins the backpropagation the training kernel is checked against: a
+    dropped term, a transposed weight or a wrong sign moves some
+    coordinate's central difference away from the analytic gradient."""
+    mlp, w, norm, x, t = _mlp_setup(seed)
+    _, g = mlp.loss_and_grad(w, norm, x, t)
+    rng = np.random.default_rng(seed + 10)
+    idx = np.concatenate([rng.choice(256, 12, replace=False),              # W1
+                          256 + rng.choice(64, 6, replace=False),          # b1
+                          320 + rng.choice(4096, 24, replace=False),       # W2
+                          4416 + rng.choice(64, 6, replace=False),         # b2
+                          4480 + rng.choice(64, 8, replace=False), [4544]])  # W3, b3
+    h = 1e-6
+    for q in idx:
+        wp, wm = w.copy(), w.copy()
+        wp[q] += h
+        wm[q] -= h
+        fd = (mlp.loss_and_grad(wp, norm, x, t)[0] - mlp.loss_and_grad(wm, norm, x, t)[0]) / (2 * h)
+        assert abs(fd - g[q]) <= 1e-6 + 1e-4 * abs(fd), (q, fd, g[q])
+
+
+def test_mlp_forward_and_loss_agree(seed=3):
+    mlp, w, norm, x, t = _mlp_setup(seed)
+    y = mlp.forward(w, norm, x)
+    loss, _ = mlp.loss_and_grad(w, norm, x, t)
+    assert np.isclose(loss, np.mean((y - t) ** 2), rtol=1e-12)
+    # prediction transform: y = t reproduces the count, clamped to [0, min(ub, n)]
+    ub = np.array([0, 10, 100, 1000])
+    cnt = np.array([0, 7, 100, 30])
+    tt = np.log1p(cnt) - np.log1p(ub)
+    p = np.where(ub > 0, np.clip(np.expm1(tt + np.log1p(ub)), 0, np.minimum(ub, 50)), 0)
+    assert np.allclose(p, [0, 7, 50, 30])
+
+
+def test_mlp_features_match_row_loop():
+    from oracle import mlp
+    rng = np.random.default_rng(4)
+    m, K, n = 30, 25, 40
+    ar = rng.integers(0, m, 200)
+    ac = rng.integers(0, K, 200)
+    blen = rng.integers(0, 9, K)
+    bptr = np.concatenate([[0], np.cumsum(blen)])
+    x, ub = mlp.features(m, ar, ac, bptr)
+    for i in range(m):
+        bs = [blen[k] for r, k in zip(ar, ac) if r == i]
+        if not bs:
+            assert np.all(x[i] == 0) and ub[i] == 0
+            continue
+        exp = np.log1p([len(bs), sum(bs), max(bs), np.std(bs)])
+        assert np.allclose(x[i], exp, rtol=1e-12, atol=1e-12)
+        assert ub[i] == sum(bs)
