@@ -1,14 +1,23 @@
 """Benchmark of the outer-product SpGEMM hot path on one MI355X.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rmat_s18_ef16]
+                [--path both|learned|plain]
 
-A step is one pass of the hot path over operands already resident in HBM:
-upper bound + tiling (InitGEMM), k-bins (CategorizeGEMM), binned expansion
-(BinGEMM), exact symbolic count (SymbolicGEMM) and merge into sorted CSR
-(MergeGEMM).  Prints one JSON line.  value = GFLOP/s with flops =
-2 * intermediate products.  The roofline object prices the expansion +
-merge kernels of one step against MI355X's 8 TB/s HBM peak with the
-algorithmic byte count
+A step is one pass of the hot path over operands already resident in HBM.
+Two paths are timed on the same product:
+
+  plain:   InitGEMM (upper bound, tiling), CategorizeGEMM (k-bins), BinGEMM
+           (binned expansion), SymbolicGEMM (exact count), MergeGEMM
+  learned: InitGEMM, CategorizeGEMM, BinGEMM, PredictRows (MLP row-count
+           prediction), LearnedMergeGEMM (merge into predicted capacities,
+           exact fallback for overflowed rows, compaction into CSR)
+
+The predictor (spgemm_amd/models/rmat_mix32.mlp, or trained first with
+--retrain, not timed) was trained on the exact counts of 32 seeded smaller
+R-MAT products.  Prints one JSON line; value = the learned path's
+GFLOP/s (flops = 2 * intermediate products), with the plain path beside it.
+The roofline object prices the expansion + merge kernels of one step
+against MI355X's 8 TB/s HBM peak with the algorithmic byte count
     16*nnz(A) + 8*nnz(B) + 16*nprod + 8*nnz(C) + 8*(m+1)
 divided by their HIP-event time on the library's stream.  cpu_baseline
 times the oracle's restatement of the reference method on a bounded row
@@ -37,6 +46,8 @@ WORKLOADS = {
     "stencil27_64": dict(kind="stencil", L=64, seed=3),
     # C = A*A, n = 4096, 8 distinct columns per row
     "uniform_4096x8": dict(kind="uniform", n=4096, k=8, seed=1),
+    # C = A*B, A = 2^20 x 2^18 R-MAT (8 * 2^20 draws), B = R-MAT scale 18, ef 8
+    "rect_rmat_20x18_ef8": dict(kind="rect", seed=4),
 }
 
 
@@ -46,6 +57,10 @@ def make_workload(name):
         A = spg.gen_rmat(w["row_bits"], w["col_bits"], w["ef"] << w["row_bits"], seed=w["seed"])
     elif w["kind"] == "stencil":
         A = spg.gen_stencil27(w["L"], w["L"], w["L"], w["seed"])
+    elif w["kind"] == "rect":
+        A = spg.gen_rmat(20, 18, 8 << 20, seed=w["seed"])
+        B = spg.gen_rmat(18, 18, 8 << 18, seed=w["seed"] + 1)
+        return A, B
     else:
         A = spg.gen_uniform_rows(w["n"], w["n"], w["k"], w["seed"])
     return A, A
@@ -58,6 +73,39 @@ def run_step(ctx):
     nnz = ctx.symbolic_gemm()
     ctx.merge_gemm()
     return nprod, nnz
+
+
+def run_step_learned(ctx, P, m):
+    nprod = ctx.init_gemm()
+    ctx.categorize_gemm()
+    ctx.bin_gemm()
+    ctx.predict_rows(P)
+    st = ctx.learned_merge_gemm()
+    return nprod, st["nnz"], st
+
+
+def timed(ctx, step, steps, warmup):
+    for _ in range(warmup):
+        step()
+    ctx.synchronize()
+    phase_sum = {p: 0.0 for p in spg.PHASES}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = step()
+        for k, v in ctx.phase_times().items():
+            phase_sum[k] += v
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    return r, ms, {k: v / steps for k, v in phase_sum.items() if v > 0}
+
+
+def roofline_of(bytes_alg, phase_ms, workload):
+    t_em = (phase_ms.get("expand", 0.0) + phase_ms.get("merge", 0.0)) * 1e-3
+    achieved = bytes_alg / t_em / 1e9 if t_em > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(workload),
+            "kernels": "expansion + merge kernels of one step (k_expand*, k_merge_*)",
+            "bytes_alg_per_step": int(bytes_alg), "kernel_ms_per_step": round(t_em * 1e3, 4)}
 
 
 def load_traffic(workload):
@@ -111,8 +159,11 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="rmat_s18_ef16", choices=sorted(WORKLOADS))
+    ap.add_argument("--path", default="both", choices=("both", "learned", "plain"))
+    ap.add_argument("--retrain", action="store_true",
+                    help="train the predictor instead of loading spgemm_amd/models/rmat_mix32.mlp")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-products", type=float, default=3e8)
+    ap.add_argument("--cpu-products", type=float, default=3e9)
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -120,54 +171,75 @@ def main():
               "C row blocks across GPUs but this build runs on a single MI355X.", file=sys.stderr)
         sys.exit(2)
 
-    A, B = make_workload(args.workload)
     ctx = spg.Context(0)
+    P = train_info = None
+    if args.path != "plain":
+        from spgemm_amd import train_predictor as tp
+        if args.retrain:
+            P, train_info = tp.train(ctx)
+        else:
+            P, train_info = tp.load_or_train(ctx)
+    A, B = make_workload(args.workload)
     ctx.set_operands_from_coo(A, B)
     m, n = A.num_rows, B.num_cols
 
-    for _ in range(args.warmup):
-        run_step(ctx)
-    ctx.synchronize()
+    res = {}
+    if args.path != "learned":
+        (nprod, nnz), ms, ph = timed(ctx, lambda: run_step(ctx), args.steps, args.warmup)
+        exact = np.diff(ctx.get_c()[0]) if args.path == "both" else None
+        res["plain"] = dict(nprod=nprod, nnz=nnz, ms=ms, phase_ms=ph)
+    if args.path != "plain":
+        (nprod, nnz, st), ms, ph = timed(ctx, lambda: run_step_learned(ctx, P, m), args.steps,
+                                         args.warmup)
+        res["learned"] = dict(nprod=nprod, nnz=nnz, ms=ms, phase_ms=ph, stats=st)
+        pred = ctx.predict_rows(P, m)
+        cnt = np.diff(ctx.get_c()[0])
+        if "plain" in res:
+            assert np.array_equal(cnt, exact), "learned and plain row counts differ"
+        live = ctx.upper_bound(m) > 0
+        relerr = np.abs(pred[live] - cnt[live]) / np.maximum(cnt[live], 1)
+        res["learned"]["rates"] = {
+            "rows": int(live.sum()),
+            "under_predicted_rows": round(float(np.mean(pred[live] < cnt[live])), 6),
+            "mean_abs_rel_err": round(float(relerr.mean()), 6),
+            "median_abs_rel_err": round(float(np.median(relerr)), 6),
+            "overflow_vrow_rate": round(st["overflow_vrows"] / max(st["vrows"], 1), 6),
+            "fallback_slot_fraction": round(st["fallback_slots"] / max(st["nnz"], 1), 6),
+            "padded_slots_per_nnz": round(st["padded_slots"] / max(st["nnz"], 1), 4),
+            "unrecovered_overflows": int(st["unrecovered"])}
 
-    phase_sum = {p: 0.0 for p in spg.PHASES}
-    nnz = 0
-    nprod = 0
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        nprod, nnz = run_step(ctx)
-        pt = ctx.phase_times()
-        for k, v in pt.items():
-            phase_sum[k] += v
-    ctx.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    phase_ms = {k: v / args.steps for k, v in phase_sum.items()}
-
+    head = res["learned"] if "learned" in res else res["plain"]
+    nprod, nnz = head["nprod"], head["nnz"]
     flops = 2.0 * nprod
-    value = flops / (ms_per_step * 1e-3) / 1e9
+    gflops = {k: flops / (r["ms"] * 1e-3) / 1e9 for k, r in res.items()}
     bytes_alg = 16 * A.nnz + 8 * B.nnz + 16 * nprod + 8 * nnz + 8 * (m + 1)
-    t_em = (phase_ms["expand"] + phase_ms["merge"]) * 1e-3
-    achieved = bytes_alg / t_em / 1e9 if t_em > 0 else 0.0
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": load_traffic(args.workload),
-                "kernels": "expansion + merge kernels of one step (k_expand, k_merge_dense, "
-                           "k_merge_hash<*>)",
-                "bytes_alg_per_step": int(bytes_alg), "kernel_ms_per_step": round(t_em * 1e3, 4)}
-
-    out = {"metric": METRIC, "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+    path_name = "learned" if "learned" in res else "plain"
+    out = {"metric": METRIC, "value": round(gflops[path_name], 2), "unit": "GFLOP/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(head["ms"], 4),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic",
-           "config": {"workload": f"{args.workload} C=A*A", "path": "plain exact-count (one product per launch sequence)",
-                      "m": m, "n": n, "nnz_A": int(A.nnz), "nnz_B": int(B.nnz),
+           "config": {"workload": f"{args.workload} C=A*B" if B is not A else f"{args.workload} C=A*A",
+                      "path": {"learned": "learned allocation: MLP row-count prediction, merge into "
+                                          "predicted capacities, exact fallback, compaction",
+                               "plain": "plain exact-count: symbolic pass then merge"}[path_name],
+                      "m": m, "k": A.num_cols, "n": n, "nnz_A": int(A.nnz), "nnz_B": int(B.nnz),
                       "nprod": int(nprod), "nnz_C": int(nnz), "parallelism": "single GPU"},
-           "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()},
-           "roofline": roofline}
+           "phase_ms": {k: round(v, 4) for k, v in head["phase_ms"].items()},
+           "roofline": roofline_of(bytes_alg, head["phase_ms"], args.workload)}
+    for k, r in res.items():
+        out[k] = {"value": round(gflops[k], 2), "ms_per_step": round(r["ms"], 4),
+                  "phase_ms": {p: round(v, 4) for p, v in r["phase_ms"].items()},
+                  "roofline_frac": roofline_of(bytes_alg, r["phase_ms"], args.workload)["frac"]}
+    if "learned" in res:
+        out["learned"]["rates"] = res["learned"]["rates"]
+        out["learned"]["predictor_training"] = train_info
+    if len(res) == 2:
+        out["learned_vs_plain"] = round(gflops["learned"] / gflops["plain"], 4)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(A, B, int(args.cpu_products), threads)
+    if P is not None:
+        P.close()
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -174,6 +174,9 @@ typedef struct spg_config {
     int32_t expand_mode;       /* SPG_EXPAND_KBINS or SPG_EXPAND_ROWS          */
     float pred_margin;         /* learned path: capacity = pred*(1+margin)+slack */
     int32_t pred_slack;        /* learned path: slots added to every capacity    */
+    int32_t pred_tight;        /* learned path: 0 = capacity is the merge class's
+                                  maximum (fewest overflows), 1 = capacity is the
+                                  inflated prediction (least memory)            */
 } spg_config;
 
 /* Expansion mappings of BinGEMM.  KBINS groups the (i,k) pairs by k as the
@@ -277,7 +280,8 @@ int spgPredictorNumParams(void);
 /* Adds the non-empty rows of ctx's last product as training samples
  * (features and exact counts); needs SymbolicGEMM or a merge on ctx. */
 int spgPredictorAddSamples(spg_predictor* p, spg_context* ctx);
-/* Adam on the mean squared error of log1p(cnt) - log1p(ub) over all
+/* Adam on the mean squared error of y against log1p(cnt) - log1p(s), with
+ * s = n * (1 - exp(-ub / n)) the count of ub uniform products, over all
  * samples; batch is rounded down to a multiple of 64.  Deterministic.  The
  * feature normalization is fixed from the samples at the first call. */
 int spgPredictorTrain(spg_predictor* p, int epochs, int batch, float lr,

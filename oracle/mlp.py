@@ -13,8 +13,10 @@ definition is the one in include/spgemm.h:
   network: h1 = relu(W1 z + b1), h2 = relu(W2 h1 + b2), y = W3 . h2 + b3,
     z = (x - mu) / sd, parameters packed as W1 (64x4, row major), b1, W2
     (64x64), b2, W3 (64), b3
-  target: t_i = log1p(nnz(C row i)) - log1p(ub_i)
-  prediction: clamp(expm1(y + log1p(ub_i)), 0, min(ub_i, n))
+  baseline: s_i = n * (1 - exp(-ub_i / n)), the distinct count of ub_i
+    products falling uniformly on n columns
+  target: t_i = log1p(nnz(C row i)) - log1p(s_i)
+  prediction: clamp(expm1(y + log1p(s_i)), 0, min(ub_i, n))
   loss: mean over samples of (y - t)^2
 
 Parity pins (tests/test_oracle.py): the gradient against central finite
@@ -68,10 +70,19 @@ def forward(w, norm, x):
     return h2 @ W3 + b3
 
 
+def log1p_uniform_count(ub, n):
+    ub = np.asarray(ub, np.float64)
+    return np.log1p(n * -np.expm1(-ub / n))
+
+
+def target(cnt, ub, n):
+    return np.log1p(np.asarray(cnt, np.float64)) - log1p_uniform_count(ub, n)
+
+
 def predict(w, norm, x, ub, n):
     y = forward(w, norm, x)
     ub = np.asarray(ub, np.float64)
-    p = np.expm1(y + np.log1p(ub))
+    p = np.expm1(y + log1p_uniform_count(ub, n))
     p = np.clip(p, 0.0, np.minimum(ub, n))
     return np.where(ub > 0, p, 0.0)
 

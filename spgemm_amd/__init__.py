@@ -68,7 +68,7 @@ class CSR64(Structure):
 class Config(Structure):
     _fields_ = [("tile_log2", c_int32), ("item_products", c_int32), ("long_row_ub", c_int64),
                 ("xcd_swizzle", c_int32), ("timing", c_int32), ("expand_mode", c_int32),
-                ("pred_margin", c_float), ("pred_slack", c_int32)]
+                ("pred_margin", c_float), ("pred_slack", c_int32), ("pred_tight", c_int32)]
 
 
 class LearnedStats(Structure):
@@ -444,9 +444,13 @@ class Context:
         return dict(zip(PHASES, ms.tolist()))
 
     # learned allocation
-    def predict_rows(self, predictor, m):
-        """Runs the MLP over the row features (after init_gemm); returns the
-        m predicted row counts and keeps them for learned_merge_gemm."""
+    def predict_rows(self, predictor, m=None):
+        """Runs the MLP over the row features (after init_gemm) and keeps the
+        predicted row counts for learned_merge_gemm; returns them (m floats)
+        when m is given, else nothing is copied to the host."""
+        if m is None:
+            self._c(lib().spgPredictRows(self.h, predictor.h, None), "spgPredictRows")
+            return None
         pred = np.zeros(max(m, 1), np.float32)
         self._c(lib().spgPredictRows(self.h, predictor.h, _ptr(pred, c_float)), "spgPredictRows")
         return pred[:m]

@@ -226,6 +226,8 @@ int spgSetConfig(spg_context* c, const spg_config* cfg) {
         return fail(c, SPG_EINVAL, "pred_margin must be in [0, 100]");
     if (cfg->pred_slack < 0 || cfg->pred_slack > (1 << 20))
         return fail(c, SPG_EINVAL, "pred_slack must be in [0, 2^20]");
+    if (cfg->pred_tight != 0 && cfg->pred_tight != 1)
+        return fail(c, SPG_EINVAL, "pred_tight must be 0 or 1");
     c->cfg = *cfg;
     c->stage = std::min(c->stage, 0);
     return SPG_OK;

@@ -1957,14 +1957,41 @@ __device__ __forceinline__ int hash_class_for(long long cap) {
     return lg - kHashMinLog2;
 }
 
-// Capacity and class of every virtual row.  A tile of a long row gets the
-// share of its row's prediction proportional to its products.
+// Distinct columns expected when u products fall uniformly on w columns.
+__device__ __forceinline__ double saturated(long long u, int w) {
+    return (double)w * -expm1(-(double)u / (double)w);
+}
+
+// Per row split into column tiles: the sum over its tiles of saturated(),
+// by which a tile's share of the row's prediction is weighted.
+__global__ void k_learned_rowest(int m, const int* __restrict__ vstart,
+                                 const long long* __restrict__ vub, const int* __restrict__ vwid,
+                                 double* __restrict__ rowest) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int s = vstart[i], e = vstart[i + 1];
+    double t = 0.0;
+    if (e - s > 1)
+        for (int v = s; v < e; ++v) t += saturated(vub[v], vwid[v]);
+    rowest[i] = t;
+}
+
+// Class and output capacity of every virtual row.  The class is chosen from
+// the row's predicted count (inflated by margin and slack); the capacity is
+// the most that class's kernel can hold (256 or 1024 columns for the wave
+// tile kernels, the hash table's half for hash rows) or the upper bound
+// min(products, width) for the dense tile and rank kernels, so a row
+// overflows only when its count exceeds its class (with tight, the inflated
+// prediction itself, which saves memory).  Capacity costs memory,
+// not bandwidth: the merge writes and the compaction reads only the row's
+// count.  A tile of a long row gets the share of its row's prediction
+// weighted by saturated(products, width).
 __global__ void k_learned_class(long long V, const int* __restrict__ vrow_row,
-                                const long long* __restrict__ vub, const int* __restrict__ vwid,
-                                const long long* __restrict__ ub, const float* __restrict__ pred,
-                                int W, int n, float margin, int slack,
-                                long long* __restrict__ vcap, int* __restrict__ cls,
-                                unsigned long long* __restrict__ nlive) {
+                                const int* __restrict__ vstart, const long long* __restrict__ vub,
+                                const int* __restrict__ vwid, const double* __restrict__ rowest,
+                                const float* __restrict__ pred, int W, int n, float margin,
+                                int slack, int tight, long long* __restrict__ vcap,
+                                int* __restrict__ cls, unsigned long long* __restrict__ nlive) {
     const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     bool live = false;
     if (v < V) {
@@ -1976,16 +2003,34 @@ __global__ void k_learned_class(long long V, const int* __restrict__ vrow_row,
             const int width = vwid[v];
             const long long bound = min(u, (long long)width);
             const int i = vrow_row[v];
-            double share = (double)pred[i] * ((double)u / (double)ub[i]);
+            double share = (double)pred[i];
+            if (vstart[i + 1] - vstart[i] > 1) {
+                const double tot = rowest[i];
+                share = tot > 0.0 ? share * saturated(u, width) / tot : 0.0;
+            }
             if (!(share > 0.0)) share = 0.0;  // negative or NaN
-            const double want = ceil(share * (1.0 + (double)margin)) + (double)slack;
-            cap = want < 1.0 ? 1 : (want >= (double)bound ? bound : (long long)want);
-            if (width <= W)
-                k = cap <= 256 ? kClassTileS : (cap <= 1024 ? kClassTileM : kClassTile);
-            else if (cap > kRankMinCount && n <= kRankMaxWords * 32)
+            double want = ceil(share * (1.0 + (double)margin)) + (double)slack;
+            want = want < 1.0 ? 1.0 : (want > (double)bound ? (double)bound : want);
+            const long long wl = (long long)want;
+            if (width <= W) {
+                if (wl <= 256) {
+                    k = kClassTileS;
+                    cap = min(bound, 256LL);
+                } else if (wl <= 1024) {
+                    k = kClassTileM;
+                    cap = min(bound, 1024LL);
+                } else {
+                    k = kClassTile;
+                    cap = bound;
+                }
+            } else if (wl > kRankMinCount && n <= kRankMaxWords * 32) {
                 k = kClassRank;
-            else
-                k = hash_class_for(cap);  // cap <= u <= 4096: at most 8192 slots
+                cap = bound;  // <= 4096 = the rank kernel's accumulator
+            } else {
+                k = hash_class_for(wl);  // wl <= u <= 4096: at most 8192 slots
+                cap = min(bound, (1LL << (k + kHashMinLog2)) / 2);
+            }
+            if (tight) cap = min(cap, wl);  // least memory, more overflows
         }
         vcap[v] = cap;
         cls[v] = k;
@@ -2056,9 +2101,25 @@ __global__ __launch_bounds__(kPT) void k_learned_compact(
             cnt = lim;
         }
         const long long o = vout[v];
-        for (long long q = lane; q < cnt; q += 64) {
-            ccol[o + q] = sc[q];
-            cval[o + q] = sv[q];
+        for (long long q0 = 0; q0 < cnt; q0 += 4 * 64) {  // four loads in flight per lane
+            int cc[4];
+            float xx[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const long long q = q0 + lane + u * 64;
+                if (q < cnt) {
+                    cc[u] = sc[q];
+                    xx[u] = sv[q];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const long long q = q0 + lane + u * 64;
+                if (q < cnt) {
+                    ccol[o + q] = cc[u];
+                    cval[o + q] = xx[u];
+                }
+            }
         }
     }
 }
@@ -2096,11 +2157,18 @@ int launch_learned_merge(spg_context* c, spg_learned_stats* st) {
     SPG_TRY(fill<unsigned long long>(c, ctr, 0ull, 64));
     SPG_TRY(fill<long long>(c, vcnt, 0, V));
     if (V > 0) {
+        // fbase doubles as the per-row tile weight sums until the fallback
+        double* rowest = (double*)fbase;
+        hipLaunchKernelGGL(k_learned_rowest, dim3((c->m + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                           c->stream, c->m, (const int*)c->vstart.p, (const long long*)c->vub.p,
+                           (const int*)c->vwid.p, rowest);
+        SPG_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_learned_class, dim3(gV), dim3(kBlock), 0, c->stream, V,
-                           (const int*)c->vrow_row.p, (const long long*)c->vub.p,
-                           (const int*)c->vwid.p, (const long long*)c->ub.p,
-                           (const float*)c->pred.p, c->W, c->n, c->cfg.pred_margin,
-                           c->cfg.pred_slack, vcap, (int*)c->vclass.p, ctr + kCtrLive);
+                           (const int*)c->vrow_row.p, (const int*)c->vstart.p,
+                           (const long long*)c->vub.p, (const int*)c->vwid.p,
+                           (const double*)rowest, (const float*)c->pred.p, c->W, c->n,
+                           c->cfg.pred_margin, c->cfg.pred_slack, c->cfg.pred_tight, vcap,
+                           (int*)c->vclass.p, ctr + kCtrLive);
         SPG_HIP(hipGetLastError());
     }
     scratch_reset(c);

@@ -5,9 +5,12 @@
 // Features of C row i (written by InitGEMM): log1p of nnz(A row i), of the
 // flop upper bound ub_i = sum over k of nnz(B row k), of max_k nnz(B row k)
 // and of the standard deviation of those nnz(B row k).  Target: the exact
-// count from the symbolic pass, as t = log1p(nnz(C row i)) - log1p(ub_i)
-// (a log ratio <= 0); the prediction is expm1(y + log1p(ub_i)) clamped to
-// [0, min(ub_i, n)].
+// count from the symbolic pass, as t = log1p(nnz(C row i)) - log1p(s_i),
+// where s_i = n * (1 - exp(-ub_i / n)) is the count expected if the row's
+// ub_i products fell uniformly on B's n columns; the network learns the
+// correction for skew and repeated columns, which keeps it usable on
+// products with other n than it was trained on.  The prediction is
+// expm1(y + log1p(s_i)) clamped to [0, min(ub_i, n)].
 //
 // Training is deterministic: a workgroup computes the per-sample
 // activations and deltas of 64 samples into LDS, then every thread sums the
@@ -35,6 +38,12 @@ constexpr int kNP = kOffB3 + 1;  // 4545 parameters
 constexpr int kSamplesPerWG = 64;
 
 __device__ __forceinline__ float relu(float x) { return x > 0.f ? x : 0.f; }
+
+// log1p of the distinct count expected for u products uniform on n columns.
+__device__ __forceinline__ float log1p_uniform_count(long long u, int n) {
+    const double s = (double)n * -expm1(-(double)u / (double)n);
+    return (float)log1p(s);
+}
 
 __device__ __forceinline__ void normalize(const float4 f, const float4 mu, const float4 sd,
                                           float x[kF]) {
@@ -82,7 +91,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_predict(const float4* __restrict
     float x[kF], h1[kH];
     normalize(feat[i], mu, sd, x);
     const float y = mlp_forward(P, x, h1);
-    float p = expm1f(y + log1pf((float)u)) * scale;
+    float p = expm1f(y + log1p_uniform_count(u, n)) * scale;
     const float hi = (float)min(u, (long long)n);
     p = p > 0.f ? p : 0.f;  // also maps NaN to 0
     p = p > hi ? hi : p;
@@ -195,7 +204,7 @@ __global__ void k_mlp_adam(float* __restrict__ P, float* __restrict__ M, float* 
 // Samples of one product: features and log-ratio targets of its non-empty
 // rows (count from the symbolic pass, crow = C's row offsets).
 __global__ void k_mlp_samples(const float4* __restrict__ feat, const long long* __restrict__ ub,
-                              const long long* __restrict__ crow, int m,
+                              const long long* __restrict__ crow, int m, int n,
                               const int* __restrict__ pos, float4* __restrict__ sfeat,
                               float* __restrict__ starget) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -203,7 +212,7 @@ __global__ void k_mlp_samples(const float4* __restrict__ feat, const long long* 
     const int q = pos[i];
     sfeat[q] = feat[i];
     const float cnt = (float)(crow[i + 1] - crow[i]);
-    starget[q] = log1pf(cnt) - log1pf((float)ub[i]);
+    starget[q] = log1pf(cnt) - log1p_uniform_count(ub[i], n);
 }
 
 __global__ void k_nonempty_flag(const long long* __restrict__ ub, int m, int* __restrict__ flag) {
@@ -374,7 +383,7 @@ int spgPredictorAddSamples(spg_predictor* p, spg_context* c) {
     if (rc != SPG_OK) return rc;
     hipLaunchKernelGGL(k_mlp_samples, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream,
                        (const float4*)c->feat.p, (const long long*)c->ub.p,
-                       (const long long*)c->crow.p, m, pos, (float4*)p->sfeat.p + n0,
+                       (const long long*)c->crow.p, m, c->n, pos, (float4*)p->sfeat.p + n0,
                        (float*)p->starget.p + n0);
     PHIP(hipGetLastError());
     std::vector<float> hf((size_t)added * 4);

@@ -22,7 +22,7 @@ PRED_RTOL = 2e-3
 GRAD_RTOL = 2e-3
 
 BASE_CFG = dict(tile_log2=13, item_products=4096, long_row_ub=4096, xcd_swizzle=1, timing=1,
-                expand_mode=0, pred_margin=0.25, pred_slack=8)
+                expand_mode=0, pred_margin=0.25, pred_slack=8, pred_tight=0)
 
 
 def gen(spg, kind, seed, **kw):
@@ -116,6 +116,7 @@ def test_inference_matches_numpy(spg, gpu_ctx):
 def _samples(spg, ctx, P, cases):
     """Runs the plain path on each case, adds its rows as samples; returns
     the host features and targets in the order the samples were added."""
+    from oracle import mlp
     xs, ts = [], []
     for kind, seed, kw in cases:
         A, sh = gen(spg, kind, seed, **kw)
@@ -125,7 +126,7 @@ def _samples(spg, ctx, P, cases):
         cnt = np.diff(C[0])
         keep = ub > 0
         xs.append(x[keep])
-        ts.append(np.log1p(cnt[keep]) - np.log1p(ub[keep]))
+        ts.append(mlp.target(cnt[keep], ub[keep], sh[1]))
     return np.concatenate(xs), np.concatenate(ts)
 
 
@@ -199,13 +200,15 @@ def test_learned_equals_plain(spg, orc, gpu_ctx, label, kind, seed, kw):
     assert st["unrecovered"] == 0 and st["nnz"] == Cp[0][-1]
 
 
+@pytest.mark.parametrize("tight", [1, 0], ids=["tight", "classmax"])
 @pytest.mark.parametrize("mode", ["zero", "huge", "random", "exact"])
 @pytest.mark.parametrize("tile_log2,long_ub", [(13, 4096), (7, 64)])
-def test_forced_predictions(spg, gpu_ctx, mode, tile_log2, long_ub):
-    """Predictions forced from the host: all zero (every row with more than
-    one column overflows and is merged again), far too large (capacities
-    at the upper bound), random around the exact count, exact."""
-    print(f"case forced predictions {mode} tile_log2={tile_log2}", flush=True)
+def test_forced_predictions(spg, gpu_ctx, mode, tile_log2, long_ub, tight):
+    """Predictions forced from the host: all zero (with tight capacities
+    every row with more than one column overflows and is merged again), far
+    too large (capacities at the upper bound), random around the exact
+    count, exact."""
+    print(f"case forced predictions {mode} tile_log2={tile_log2} tight={tight}", flush=True)
     A, sh = gen(spg, "rmat", 6, s=12, ef=12)
     cfg = dict(tile_log2=tile_log2, long_row_ub=long_ub)
     Cp = plain(spg, gpu_ctx, A, sh, A, sh, cfg)
@@ -213,20 +216,25 @@ def test_forced_predictions(spg, gpu_ctx, mode, tile_log2, long_ub):
     rng = np.random.default_rng(0)
     pred = {"zero": np.zeros(sh[0]), "huge": np.full(sh[0], 1e12),
             "random": cnt * rng.uniform(0.0, 2.0, sh[0]), "exact": cnt}[mode]
-    lcfg = dict(cfg, pred_margin=0.0, pred_slack=0) if mode != "huge" else cfg
+    lcfg = dict(cfg, pred_tight=tight)
+    if mode != "huge":
+        lcfg.update(pred_margin=0.0, pred_slack=0)
     Cl, st = learned(spg, gpu_ctx, A, sh, A, sh, pred=pred.astype(np.float32), cfg=lcfg)
     print(f"  stats {st}")
     same_c(Cl, Cp)
     assert st["unrecovered"] == 0
     if mode == "zero":
-        assert st["overflow_vrows"] > 0.5 * st["vrows"]
+        if tight:  # capacity 1: every row with two or more columns
+            assert st["overflow_vrows"] > 0.5 * st["vrows"]
+        elif tile_log2 == 13:  # one tile per row: the 256-column wave class
+            assert st["overflow_vrows"] == int(np.sum(cnt > 256))
     if mode == "huge" or (mode == "exact" and tile_log2 == 13):
         assert st["overflow_vrows"] == 0
     if mode == "exact" and tile_log2 < 13:
-        # a row's exact count is shared among its column tiles in
-        # proportion to their products, so some tiles still overflow
+        # a row's exact count is shared among its column tiles by weight,
+        # so with tight capacities some tiles still overflow
         assert st["overflow_vrows"] < 0.5 * st["vrows"]
-    if mode == "random":
+    if mode == "random" and tight:
         assert 0 < st["overflow_vrows"] < st["vrows"]
 
 
@@ -246,7 +254,7 @@ def test_forced_overflow_wide_hash(spg, gpu_ctx):
     Cp = plain(spg, gpu_ctx, A, (m, K), B, (K, n))
     for pred in (np.zeros(m), np.full(m, 200.0), np.diff(Cp[0]).astype(np.float64)):
         Cl, st = learned(spg, gpu_ctx, A, (m, K), B, (K, n), pred=pred.astype(np.float32),
-                         cfg=dict(pred_margin=0.0, pred_slack=0))
+                         cfg=dict(pred_margin=0.0, pred_slack=0, pred_tight=1))
         print(f"  stats {st}")
         same_c(Cl, Cp)
         assert st["unrecovered"] == 0

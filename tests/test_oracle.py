@@ -224,12 +224,17 @@ def test_mlp_forward_and_loss_agree(seed=3):
     y = mlp.forward(w, norm, x)
     loss, _ = mlp.loss_and_grad(w, norm, x, t)
     assert np.isclose(loss, np.mean((y - t) ** 2), rtol=1e-12)
-    # prediction transform: y = t reproduces the count, clamped to [0, min(ub, n)]
+    # prediction transform: y = target reproduces the count, clamped to
+    # [0, min(ub, n)]; the uniform baseline is ub for ub << n and n for ub >> n
     ub = np.array([0, 10, 100, 1000])
     cnt = np.array([0, 7, 100, 30])
-    tt = np.log1p(cnt) - np.log1p(ub)
-    p = np.where(ub > 0, np.clip(np.expm1(tt + np.log1p(ub)), 0, np.minimum(ub, 50)), 0)
+    n = 50
+    tt = mlp.target(cnt, ub, n)
+    base = mlp.log1p_uniform_count(ub, n)
+    p = np.where(ub > 0, np.clip(np.expm1(tt + base), 0, np.minimum(ub, n)), 0)
     assert np.allclose(p, [0, 7, 50, 30])
+    assert np.isclose(mlp.log1p_uniform_count(1, 1e9), np.log1p(1.0), rtol=1e-6)
+    assert np.isclose(mlp.log1p_uniform_count(1e6, 50), np.log1p(50.0))
 
 
 def test_mlp_features_match_row_loop():
