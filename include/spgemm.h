@@ -308,6 +308,28 @@ int spgLearnedMergeGEMM(spg_context* ctx, spg_learned_stats* st);
 int spgMultiplyLearned(spg_context* ctx, const spg_predictor* p,
                        spg_learned_stats* st);
 
+/* ---------------- batched products (no reference counterpart) -----------
+ * count independent products C_b = A_b * B_b run as one: the operands are
+ * stacked block-diagonally (product b's columns start at a multiple of
+ * 8192) and every phase kernel runs once over the work of all products. */
+#define SPG_BATCH_GRAPH 1  /* record the run once, then replay it as one HIP graph */
+
+/* A[b]: CSC (COL_MAJOR), B[b]: CSR (ROW_MAJOR), host arrays (copied). */
+int spgBatchSetOperands(spg_context* ctx, int count, const cm* A, const cm* B);
+/* row_off (count+1 entries): product b's C rows in the stacked C;
+ * col_off (count): its column offset there.  Either may be NULL. */
+int spgBatchGetLayout(spg_context* ctx, int64_t* row_off, int64_t* col_off);
+/* Plain path (p == NULL) or learned path over the whole batch.  With
+ * SPG_BATCH_GRAPH the first call runs normally while recording its host
+ * readbacks and captures the same sequence into a HIP graph; later calls
+ * with the same operands, config and predictor (weights unchanged) replay
+ * it with one hipGraphLaunch and no host synchronization (call
+ * spgSynchronize before reading C).  st (learned path; may be NULL). */
+int spgBatchMultiply(spg_context* ctx, const spg_predictor* p, int flags,
+                     spg_learned_stats* st);
+/* Host copy of product b's C (columns relative to its block). */
+int spgBatchGetC(spg_context* ctx, int b, spg_csr64* out);
+
 #ifdef __cplusplus
 }
 #endif

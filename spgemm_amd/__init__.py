@@ -35,6 +35,7 @@ PHASES = ("init", "categorize", "expand", "symbolic", "merge", "predict", "compa
 MTX_COMPAT_VALUES = 1
 EXPAND_KBINS = 0
 EXPAND_ROWS = 1
+BATCH_GRAPH = 1
 
 STATUS = {0: "SPG_OK", 1: "SPG_EINVAL", 2: "SPG_ENOMEM", 3: "SPG_EHIP", 4: "SPG_EORDER",
           5: "SPG_EIO", 6: "SPG_ENODEV", 7: "SPG_EOVERFLOW"}
@@ -149,6 +150,10 @@ _SIGS = {
     "spgSetRowPredictions": (c_int, [c_void_p, POINTER(c_float)]),
     "spgLearnedMergeGEMM": (c_int, [c_void_p, POINTER(LearnedStats)]),
     "spgMultiplyLearned": (c_int, [c_void_p, c_void_p, POINTER(LearnedStats)]),
+    "spgBatchSetOperands": (c_int, [c_void_p, c_int, POINTER(CM), POINTER(CM)]),
+    "spgBatchGetLayout": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64)]),
+    "spgBatchMultiply": (c_int, [c_void_p, c_void_p, c_int, POINTER(LearnedStats)]),
+    "spgBatchGetC": (c_int, [c_void_p, c_int, POINTER(CSR64)]),
 }
 
 
@@ -463,6 +468,48 @@ class Context:
         st = LearnedStats()
         self._c(lib().spgLearnedMergeGEMM(self.h, byref(st)), "spgLearnedMergeGEMM")
         return {f: getattr(st, f) for f, _ in LearnedStats._fields_}
+
+    # batched products
+    def batch_set_operands(self, pairs):
+        """pairs: list of (A, B) HostCOO operands; A is converted to CSC and
+        B to CSR on the host, then the batch is stacked on the GPU."""
+        n = len(pairs)
+        As, Bs = (CM * n)(), (CM * n)()
+        keep = []
+        for b, (A, B) in enumerate(pairs):
+            for arr, M, typ in ((As, A, COL_MAJOR), (Bs, B, ROW_MAJOR)):
+                ptr, idx, val = M.to_compressed(typ)
+                keep += [ptr, idx, val]
+                x = arr[b]
+                x.ptr, x.idx, x.val = _ptr(ptr, c_int), _ptr(idx, c_int), _ptr(val, c_float)
+                x.nnz, x.num_rows, x.num_cols, x.type = M.nnz, M.num_rows, M.num_cols, bool(typ)
+        self._c(lib().spgBatchSetOperands(self.h, n, As, Bs), "spgBatchSetOperands")
+        self.batch_size = n
+
+    def batch_layout(self):
+        n = self.batch_size
+        ro = np.zeros(n + 1, np.int64)
+        co = np.zeros(n, np.int64)
+        self._c(lib().spgBatchGetLayout(self.h, _ptr(ro, c_int64), _ptr(co, c_int64)),
+                "spgBatchGetLayout")
+        return ro, co
+
+    def batch_multiply(self, predictor=None, graph=False):
+        st = LearnedStats()
+        self._c(lib().spgBatchMultiply(self.h, predictor.h if predictor else None,
+                                       BATCH_GRAPH if graph else 0, byref(st)),
+                "spgBatchMultiply")
+        return {f: getattr(st, f) for f, _ in LearnedStats._fields_} if predictor else None
+
+    def batch_get_c(self, b):
+        out = CSR64()
+        self._c(lib().spgBatchGetC(self.h, b, byref(out)), "spgBatchGetC")
+        m, nnz = out.num_rows, out.nnz
+        ptr = np.ctypeslib.as_array(out.ptr, shape=(m + 1,)).copy()
+        idx = np.ctypeslib.as_array(out.idx, shape=(max(nnz, 1),))[:nnz].copy()
+        val = np.ctypeslib.as_array(out.val, shape=(max(nnz, 1),))[:nnz].copy()
+        lib().spgFreeCSR64(byref(out))
+        return ptr, idx, val
 
     def multiply_learned(self, predictor):
         st = LearnedStats()

@@ -73,7 +73,7 @@ void* scratch_alloc(spg_context* c, size_t bytes) {
 void scratch_reset(spg_context* c) { c->scratch_top = 0; }
 
 void tmark(spg_context* c, int phase, int end) {
-    if (!c->cfg.timing) return;
+    if (!c->cfg.timing || c->plan_mode == 2) return;  // no event nodes in a captured run
     (void)hipEventRecord(c->timer.ev[2 * phase + end], c->stream);
     if (end) c->timer.recorded[phase] = true;
 }
@@ -85,11 +85,27 @@ static int copy_d2h(spg_context* c, void* dst, const void* src, size_t bytes) {
     return SPG_OK;
 }
 
+int readback(spg_context* c, const void* d, size_t bytes, void* h) {
+    if (bytes > c->pinned_small.cap) return fail(c, SPG_EINVAL, "readback too large");
+    if (c->plan_mode == 2) {
+        if (c->plan_pos + bytes > c->plan.size())
+            return fail(c, SPG_EINVAL, "replay log exhausted (batch changed since it was recorded)");
+        memcpy(h, c->plan.data() + c->plan_pos, bytes);
+        c->plan_pos += bytes;
+        return SPG_OK;
+    }
+    SPG_TRY(copy_d2h(c, c->pinned_small.p, d, bytes));
+    memcpy(h, c->pinned_small.p, bytes);
+    if (c->plan_mode == 1) {
+        const unsigned char* b = (const unsigned char*)h;
+        c->plan.insert(c->plan.end(), b, b + bytes);
+    }
+    return SPG_OK;
+}
+
 template <typename T>
 static int read_scalar(spg_context* c, const void* src, T* out) {
-    SPG_TRY(copy_d2h(c, c->pinned_small.p, src, sizeof(T)));
-    *out = *(T*)c->pinned_small.p;
-    return SPG_OK;
+    return readback(c, src, sizeof(T), out);
 }
 
 static void operand_release(Operand& o) {
@@ -177,6 +193,7 @@ int spgDestroy(spg_context* c) {
     if (!c) return SPG_EINVAL;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    graph_release(c);
     operand_release(c->A);
     operand_release(c->B);
     DevBuf* bufs[] = {&c->ub, &c->ntile, &c->vstart, &c->vrow_row, &c->vlo, &c->vwid,
@@ -187,7 +204,7 @@ int spgDestroy(spg_context* c) {
                       &c->kitem_off, &c->icol, &c->ival, &c->vclass, &c->vlist,
                       &c->vclass_off, &c->vcnt, &c->vout, &c->crow, &c->ccol, &c->cval,
                       &c->pred, &c->vcap, &c->pbase, &c->fbase, &c->pcol, &c->pval,
-                      &c->fcol, &c->fval, &c->scratch};
+                      &c->fcol, &c->fval, &c->rwinb, &c->kwinb, &c->scratch};
     for (DevBuf* b : bufs) release(*b);
     if (c->pinned_small.p) (void)hipHostFree(c->pinned_small.p);
     for (auto& e : c->timer.ev)
@@ -200,6 +217,7 @@ int spgDestroy(spg_context* c) {
 int spgSetStream(spg_context* c, void* stream) {
     if (!c) return SPG_EINVAL;
     SPG_HIP(hipStreamSynchronize(c->stream));
+    graph_release(c);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     c->stream = (hipStream_t)stream;
     c->own_stream = false;
@@ -230,6 +248,7 @@ int spgSetConfig(spg_context* c, const spg_config* cfg) {
         return fail(c, SPG_EINVAL, "pred_tight must be 0 or 1");
     c->cfg = *cfg;
     c->stage = std::min(c->stage, 0);
+    graph_release(c);
     return SPG_OK;
 }
 
@@ -241,6 +260,11 @@ int spgSynchronize(spg_context* c) {
 
 static int set_operand_device_arrays(spg_context* c, int which, int nrows, int ncols, int nnz) {
     Operand& o = which == SPG_OPERAND_A ? c->A : c->B;
+    // a new operand ends batch mode (spgBatchSetOperands sets it again after)
+    c->rwin = nullptr;
+    c->kwin = nullptr;
+    c->batch_count = 0;
+    graph_release(c);
     o.num_rows = nrows;
     o.num_cols = ncols;
     o.nnz = nnz;
@@ -380,6 +404,10 @@ int spgInitGEMM(spg_context* c, cm* C, int64_t* nprod) {
     c->n = c->B.num_cols;
     c->W = 1 << c->cfg.tile_log2;
     c->T = std::max(1, (c->n + c->W - 1) / c->W);
+    if (c->batch_count > 0) {  // tiles of the widest product's column window
+        c->T = 1;
+        for (int32_t w : c->b_ncols) c->T = std::max(c->T, (w + c->W - 1) / c->W);
+    }
     if (c->T > 1024)
         return fail(c, SPG_EINVAL, "B has more than 1024 column tiles; raise tile_log2");
     for (auto& r : c->timer.recorded) r = false;

@@ -357,6 +357,8 @@ __global__ __launch_bounds__(kBlock) void k_row_bound(const int* __restrict__ rp
                                                        const int* __restrict__ rk,
                                                        const int* __restrict__ b_ptr, int m,
                                                        long long long_ub, int T,
+                                                       const int2* __restrict__ rwin,
+                                                       int tile_log2,
                                                        long long* __restrict__ ub,
                                                        int* __restrict__ ntile,
                                                        float4* __restrict__ feat) {
@@ -377,7 +379,9 @@ __global__ __launch_bounds__(kBlock) void k_row_bound(const int* __restrict__ rp
     mx = wave_max(mx);
     if (lane_id() == 0) {
         ub[i] = s;
-        ntile[i] = (T > 1 && s > long_ub) ? T : 1;
+        // tiles of the row's column window (batched products: its block)
+        const int Ti = rwin ? (rwin[i].y + (1 << tile_log2) - 1) >> tile_log2 : T;
+        ntile[i] = (Ti > 1 && s > long_ub) ? Ti : 1;
         const int r = e1 - e0;
         double var = 0.0;
         if (r > 0) {
@@ -390,39 +394,46 @@ __global__ __launch_bounds__(kBlock) void k_row_bound(const int* __restrict__ rp
     }
 }
 
+// Column range of every virtual row: the row's window [lo, lo + w) (all of
+// B's columns for one product; the product's block in a batch), or one
+// W-wide tile of it.
 __global__ void k_fill_vrows(const int* __restrict__ vstart, const long long* __restrict__ ub,
-                             int m, int n, int tile_log2, int* __restrict__ vrow_row,
-                             long long* __restrict__ vub, int* __restrict__ vlo,
-                             int* __restrict__ vwid) {
+                             int m, int n, const int2* __restrict__ rwin, int tile_log2,
+                             int* __restrict__ vrow_row, long long* __restrict__ vub,
+                             int* __restrict__ vlo, int* __restrict__ vwid) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const int s = vstart[i], e = vstart[i + 1];
+    const int wlo = rwin ? rwin[i].x : 0, ww = rwin ? rwin[i].y : n;
     for (int v = s; v < e; ++v) {
         vrow_row[v] = i;
         if (e - s == 1) {
             vub[v] = ub[i];
-            vlo[v] = 0;
-            vwid[v] = n;
+            vlo[v] = wlo;
+            vwid[v] = ww;
         } else {
             const int lo = (v - s) << tile_log2;
-            vlo[v] = lo;
-            vwid[v] = min(1 << tile_log2, n - lo);
+            vlo[v] = wlo + lo;
+            vwid[v] = min(1 << tile_log2, ww - lo);
         }
     }
 }
 
-// Split points of every B row at the tile boundaries t*W (t = 0..T).
+// Split points of every B row at the tile boundaries lo + t*W of its column
+// window (t = 0..T; past the window's last tile: the row's end).
 __global__ void k_bsplit(const int* __restrict__ b_ptr, const int* __restrict__ b_idx, int K,
-                         int T, int tile_log2, int* __restrict__ bsplit) {
+                         int T, const int2* __restrict__ kwin, int tile_log2,
+                         int* __restrict__ bsplit) {
     long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (long long)K * (T + 1)) return;
     const int k = (int)(q / (T + 1)), t = (int)(q - (long long)k * (T + 1));
     int lo = b_ptr[k], hi = b_ptr[k + 1];
-    if (t == T) {
+    const int Tk = kwin ? (kwin[k].y + (1 << tile_log2) - 1) >> tile_log2 : T;
+    if (t >= Tk) {
         bsplit[q] = hi;
         return;
     }
-    const long long bound = (long long)t << tile_log2;
+    const long long bound = (kwin ? (long long)kwin[k].x : 0LL) + ((long long)t << tile_log2);
     while (lo < hi) {
         int mid = lo + ((hi - lo) >> 1);
         if (b_idx[mid] < bound) lo = mid + 1;
@@ -444,14 +455,15 @@ __global__ __launch_bounds__(kBlock) void k_row_tile_bound(const int* __restrict
     __shared__ long long part[kBlock / 64][64];
     const int i = blockIdx.x;
     const int vs = vstart[i];
-    if (vstart[i + 1] - vs == 1) return;
+    const int Ti = vstart[i + 1] - vs;  // the row's tiles (T is the split-point stride - 1)
+    if (Ti == 1) return;
     const int wv = threadIdx.x >> 6, lane = lane_id();
     constexpr int NWV = kBlock / 64;
     const int e0 = rptr[i], e1 = rptr[i + 1];
-    for (int t0 = 0; t0 < T; t0 += 64) {
+    for (int t0 = 0; t0 < Ti; t0 += 64) {
         const int t = t0 + lane;
         long long s = 0;
-        if (t < T) {
+        if (t < Ti) {
             int e = e0 + wv;
             for (; e + 3 * NWV < e1; e += 4 * NWV) {
                 int kk[4];
@@ -473,7 +485,7 @@ __global__ __launch_bounds__(kBlock) void k_row_tile_bound(const int* __restrict
         }
         part[wv][lane] = s;
         __syncthreads();
-        if (wv == 0 && t < T) {
+        if (wv == 0 && t < Ti) {
             long long tot = 0;
 #pragma unroll
             for (int q = 0; q < NWV; ++q) tot += part[q][lane];
@@ -510,8 +522,9 @@ int launch_init(spg_context* c) {
         const long long nbm = ((long long)m * 64 + kBlock - 1) / kBlock;
         hipLaunchKernelGGL(k_row_bound, dim3((unsigned)nbm), dim3(kBlock), 0, c->stream,
                            (const int*)rptr, (const int*)rk, c->B.ptr, m,
-                           (long long)c->cfg.long_row_ub, c->T, (long long*)c->ub.p,
-                           (int*)c->ntile.p, (float4*)c->feat.p);
+                           (long long)c->cfg.long_row_ub, c->T, (const int2*)c->rwin,
+                           c->cfg.tile_log2, (long long*)c->ub.p, (int*)c->ntile.p,
+                           (float4*)c->feat.p);
         SPG_HIP(hipGetLastError());
     }
     return scan_i32(c, (const int32_t*)c->ntile.p, (int32_t*)c->vstart.p, m);
@@ -522,15 +535,15 @@ int launch_init_vrows(spg_context* c) {
     if (m > 0) {
         hipLaunchKernelGGL(k_fill_vrows, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0,
                            c->stream, (const int*)c->vstart.p, (const long long*)c->ub.p, m,
-                           c->n, c->cfg.tile_log2, (int*)c->vrow_row.p, (long long*)c->vub.p,
-                           (int*)c->vlo.p, (int*)c->vwid.p);
+                           c->n, (const int2*)c->rwin, c->cfg.tile_log2, (int*)c->vrow_row.p,
+                           (long long*)c->vub.p, (int*)c->vlo.p, (int*)c->vwid.p);
         SPG_HIP(hipGetLastError());
     }
     if (c->have_long && K > 0) {
         long long nq = (long long)K * (c->T + 1);
         hipLaunchKernelGGL(k_bsplit, dim3((unsigned)((nq + kBlock - 1) / kBlock)), dim3(kBlock),
-                           0, c->stream, c->B.ptr, c->B.idx, K, c->T, c->cfg.tile_log2,
-                           (int*)c->bsplit.p);
+                           0, c->stream, c->B.ptr, c->B.idx, K, c->T, (const int2*)c->kwin,
+                           c->cfg.tile_log2, (int*)c->bsplit.p);
         SPG_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_row_tile_bound, dim3((unsigned)m), dim3(kBlock), 0, c->stream,
                            (const int*)c->arow_ptr.p, (const int*)c->arow_k.p,
@@ -597,10 +610,8 @@ int launch_categorize(spg_context* c) {
     }
     SPG_TRY(stable_partition(c, (const int32_t*)c->kbin.p, K, SPG_NUM_KBINS,
                              (int32_t*)c->kperm.p, (int64_t*)c->kbin_off.p));
-    long long* off_h = (long long*)c->pinned_small.p;
-    SPG_HIP(hipMemcpyAsync(off_h, c->kbin_off.p, sizeof(long long) * (SPG_NUM_KBINS + 1),
-                           hipMemcpyDeviceToHost, c->stream));
-    SPG_HIP(hipStreamSynchronize(c->stream));
+    long long off_h[SPG_NUM_KBINS + 1];
+    SPG_TRY(readback(c, c->kbin_off.p, sizeof(off_h), off_h));
     for (int b = 0; b <= SPG_NUM_KBINS; ++b) c->kbin_off_h[b] = off_h[b];
     const long long nk = c->kbin_off_h[SPG_NUM_KBINS];
     if (nk > 0) {
@@ -615,9 +626,7 @@ int launch_categorize(spg_context* c) {
     hipLaunchKernelGGL(k_gather_bin_items, dim3(1), dim3(64), 0, c->stream,
                        (const long long*)c->kbin_off.p, (const long long*)c->kitem_off.p, dtmp);
     SPG_HIP(hipGetLastError());
-    SPG_HIP(hipMemcpyAsync(off_h, dtmp, sizeof(long long) * (SPG_NUM_KBINS + 1),
-                           hipMemcpyDeviceToHost, c->stream));
-    SPG_HIP(hipStreamSynchronize(c->stream));
+    SPG_TRY(readback(c, dtmp, sizeof(off_h), off_h));
     for (int b = 0; b <= SPG_NUM_KBINS; ++b) c->kitems_bin_h[b] = off_h[b];
     return SPG_OK;
 }
@@ -664,6 +673,7 @@ struct ExpandArgs {
     const long long* vbase;
     int* cursor;
     const int* bsplit;
+    const int2* kwin;  // column window of every k (batched products), or null
     int T, tile_log2;
     int swizzle;
     int* icol;
@@ -687,6 +697,7 @@ __device__ void expand_work(const ExpandArgs& g, ExpandLds& L, int k, long long 
     int NE = kBlock >> lg;
     const int T = g.T;
     const int b0 = g.b_ptr[k];
+    const int tfirst = g.kwin ? (g.kwin[k].x >> g.tile_log2) : 0;  // windows are W-aligned
     __syncthreads();  // LDS of the previous item is free
     if (T > 1) {
         const int cap = kTBCap / T;
@@ -749,7 +760,7 @@ __device__ void expand_work(const ExpandArgs& g, ExpandLds& L, int k, long long 
                     if (jj < nj) {
                         long long slot;
                         if (!fl) slot = base + jj;
-                        else slot = L.tbase[q * T + (col[u] >> g.tile_log2)] + jlo + jj;
+                        else slot = L.tbase[q * T + (col[u] >> g.tile_log2) - tfirst] + jlo + jj;
                         icol[slot] = col[u];
                         ival[slot] = __fmul_rn(a, bv[u]);
                     }
@@ -843,6 +854,7 @@ static ExpandArgs expand_args(spg_context* c) {
     g.vbase = (const long long*)c->vbase.p;
     g.cursor = (int*)c->cursor.p;
     g.bsplit = (const int*)c->bsplit.p;
+    g.kwin = (const int2*)c->kwin;
     g.T = c->have_long ? c->T : 1;
     g.tile_log2 = c->cfg.tile_log2;
     g.swizzle = c->cfg.xcd_swizzle;
@@ -1096,10 +1108,8 @@ int launch_expand_rows(spg_context* c) {
     SPG_HIP(hipGetLastError());
     SPG_TRY(stable_partition(c, (const int32_t*)c->vclass.p, V, 2, (int32_t*)c->vlist.p,
                              (int64_t*)c->vclass_off.p));
-    long long* off_h = (long long*)c->pinned_small.p;
-    SPG_HIP(hipMemcpyAsync(off_h, c->vclass_off.p, sizeof(long long) * 3, hipMemcpyDeviceToHost,
-                           c->stream));
-    SPG_HIP(hipStreamSynchronize(c->stream));
+    long long off_h[3];
+    SPG_TRY(readback(c, c->vclass_off.p, sizeof(off_h), off_h));
     const long long n0 = off_h[1] - off_h[0], n1 = off_h[2] - off_h[1];
     RowsArgs g{};
     g.rptr = (const int*)c->arow_ptr.p;
@@ -1328,12 +1338,7 @@ static void launch_sym_hash(spg_context* c, const int* list, long long cnt) {
 }
 
 static int class_offsets(spg_context* c, long long* off) {
-    long long* off_h = (long long*)c->pinned_small.p;
-    SPG_HIP(hipMemcpyAsync(off_h, c->vclass_off.p, sizeof(long long) * (kNumVClasses + 1),
-                           hipMemcpyDeviceToHost, c->stream));
-    SPG_HIP(hipStreamSynchronize(c->stream));
-    for (int b = 0; b <= kNumVClasses; ++b) off[b] = off_h[b];
-    return SPG_OK;
+    return readback(c, c->vclass_off.p, sizeof(long long) * (kNumVClasses + 1), off);
 }
 
 int launch_symbolic(spg_context* c) {
@@ -1411,7 +1416,7 @@ __global__ void k_vclass_merge(long long V, const long long* __restrict__ vcnt,
         cls[v] = cnt <= 256 ? kClassTileS : (cnt <= 1024 ? kClassTileM : kClassTile);
         return;
     }
-    if (cnt > kRankMinCount && n <= kRankMaxWords * 32 && cnt <= kHashMaxCount) {
+    if (cnt > kRankMinCount && width <= kRankMaxWords * 32 && cnt <= kHashMaxCount) {
         cls[v] = kClassRank;
         return;
     }
@@ -1752,7 +1757,8 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
 // columns are emitted from the bitmap in ascending order (no sort).
 __global__ __launch_bounds__(kPT) void k_merge_rank(
     const int* __restrict__ vlist, long long nlist, unsigned long long* __restrict__ ctr,
-    const long long* __restrict__ vbase, int n, const int* __restrict__ icol,
+    const long long* __restrict__ vbase, const int* __restrict__ vlo,
+    const int* __restrict__ vwid, const int* __restrict__ icol,
     const float* __restrict__ ival, const long long* __restrict__ vout,
     int* __restrict__ ccol, float* __restrict__ cval, long long* __restrict__ cnt_out) {
     constexpr int WPT = kRankMaxWords / kPT;  // 16 words per thread
@@ -1761,7 +1767,6 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
     __shared__ double acc[kHashMaxCount];
     __shared__ int sh[kPT / 64];
     __shared__ long long sj;
-    const int nw = (n + 31) >> 5;
     const int w0 = threadIdx.x * WPT;
     for (int w = threadIdx.x; w < kRankMaxWords; w += kPT) bits[w] = 0u;
     for (;;) {
@@ -1771,13 +1776,15 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
         const long long s0 = vbase[v], s1 = vbase[v + 1];
         const long long o = vout[v];
         const int cnt = (int)min((long long)kHashMaxCount, vout[v + 1] - o);
+        const int lo = vlo[v];                  // columns of the row: [lo, lo + width)
+        const int nw = (vwid[v] + 31) >> 5;     // width <= 2^18 (class condition)
         // trip count uniform across the workgroup (the OR below shuffles)
         for (long long sb = s0; sb < s1; sb += 4 * kPT) {
             int c[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const long long q = sb + threadIdx.x + (long long)u * kPT;
-                c[u] = q < s1 ? icol[q] : -1;
+                c[u] = q < s1 ? icol[q] - lo : -1;
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) wave_bitmap_or(bits, c[u]);
@@ -1805,7 +1812,7 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
                 c[u] = -1;
                 x[u] = 0.f;
                 if (q < s1) {
-                    c[u] = icol[q];
+                    c[u] = icol[q] - lo;
                     x[u] = ival[q];
                 }
             }
@@ -1829,7 +1836,7 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
                 const int bit = __ffs(b) - 1;
                 b &= b - 1;
                 if (r < cnt) {
-                    ccol[o + r] = (w << 5) + bit;
+                    ccol[o + r] = lo + (w << 5) + bit;
                     cval[o + r] = (float)acc[r];
                 }
                 ++r;
@@ -1890,8 +1897,8 @@ static int launch_merge_class(spg_context* c, int b, const int* list, long long 
                            o.obase, o.ocol, o.oval, o.cnt);
     } else if (b == kClassRank) {
         hipLaunchKernelGGL(k_merge_rank, dim3(persistent_grid(c, cnt, 2)), dim3(kPT), 0,
-                           c->stream, list, cnt, ctr + 2, vbase, c->n, icol, ival, o.obase, o.ocol,
-                           o.oval, o.cnt);
+                           c->stream, list, cnt, ctr + 2, vbase, vlo, (const int*)c->vwid.p, icol,
+                           ival, o.obase, o.ocol, o.oval, o.cnt);
     } else {
         switch (b + kHashMinLog2) {
             case 6: launch_merge_hash<6>(c, list, cnt, o); break;
@@ -2023,7 +2030,7 @@ __global__ void k_learned_class(long long V, const int* __restrict__ vrow_row,
                     k = kClassTile;
                     cap = bound;
                 }
-            } else if (wl > kRankMinCount && n <= kRankMaxWords * 32) {
+            } else if (wl > kRankMinCount && width <= kRankMaxWords * 32) {
                 k = kClassRank;
                 cap = bound;  // <= 4096 = the rank kernel's accumulator
             } else {
@@ -2058,7 +2065,7 @@ __global__ void k_learned_check(long long V, const long long* __restrict__ vcnt,
             f = min(vub[v], (long long)width);
             if (width <= W)
                 k = kClassTile;
-            else if (n <= kRankMaxWords * 32)
+            else if (width <= kRankMaxWords * 32)
                 k = kClassRank;
             else
                 k = hash_class_for(f);
@@ -2128,12 +2135,9 @@ __global__ __launch_bounds__(kPT) void k_learned_compact(
 constexpr int kCtrRank = 2, kCtrLive = 8, kCtrOver = 9, kCtrUnrec = 10;
 
 static int readback2(spg_context* c, const void* a, const void* b, long long* out) {
-    long long* h = (long long*)c->pinned_small.p;
-    SPG_HIP(hipMemcpyAsync(h, a, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
-    if (b) SPG_HIP(hipMemcpyAsync(h + 1, b, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
-    SPG_HIP(hipStreamSynchronize(c->stream));
-    out[0] = h[0];
-    out[1] = b ? h[1] : 0;
+    SPG_TRY(readback(c, a, sizeof(long long), out));
+    out[1] = 0;
+    if (b) SPG_TRY(readback(c, b, sizeof(long long), out + 1));
     return SPG_OK;
 }
 

@@ -53,6 +53,12 @@ struct spg_context {
     std::string err;
 
     spg::Operand A, B;  // A: CSC (ptr over columns), B: CSR (ptr over rows)
+    // Batched products (block-diagonal A and B): the column window (lo,
+    // width) of every C row and of every k, or null for one product (all of
+    // B's columns).  Window starts are multiples of 2^kTileMaxLog2.
+    spg::DevBuf rwinb, kwinb;
+    void* rwin = nullptr;
+    void* kwin = nullptr;
 
     // shape of the product
     int32_t m = 0, K = 0, n = 0;
@@ -92,6 +98,29 @@ struct spg_context {
 
     spg::Timer timer;
     double phase_ms[SPG_NUM_PHASES] = {};
+
+    // Replay log.  A recorded run appends every host readback (sizes, class
+    // offsets) to plan; a replayed run (the one captured into a HIP graph)
+    // takes them from plan instead of reading the device, so it contains no
+    // synchronization.  Valid while the operands' structure, the config and
+    // the predictor are unchanged.
+    int plan_mode = 0;  // 0: off, 1: record, 2: replay
+    std::vector<unsigned char> plan;
+    size_t plan_pos = 0;
+
+    // Batch of products stacked block-diagonally (spgBatchSetOperands):
+    // product b owns C rows [row_off[b], row_off[b+1]), k [k_off[b],
+    // k_off[b+1]) and columns [col_off[b], col_off[b] + ncols[b]).
+    int batch_count = 0;
+    std::vector<int64_t> b_row_off, b_k_off, b_col_off;
+    std::vector<int32_t> b_ncols;
+    // captured batch run
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_learned = false;
+    const spg_predictor* graph_pred = nullptr;
+    long long graph_pred_version = -1;
+    spg_learned_stats graph_stats{};
 };
 
 namespace spg {
@@ -103,6 +132,11 @@ void scratch_reset(spg_context* c);
 int fail(spg_context* c, int code, const std::string& msg);
 // Records the start (end = 0) or end (end = 1) event of a phase timer.
 void tmark(spg_context* c, int phase, int end);
+// Copies bytes (<= 4096) from device memory to h, synchronizing the
+// context stream; recorded into / replayed from the context's replay log.
+int readback(spg_context* c, const void* d, size_t bytes, void* h);
+// Drops a captured batch graph (operands, config or predictor changed).
+void graph_release(spg_context* c);
 // Fill kernel (used instead of hipMemsetAsync so the path stays capturable).
 template <typename T>
 int fill(spg_context* c, T* p, T v, long long n);
@@ -131,6 +165,7 @@ int launch_rowptr(spg_context* c);
 // Runs the MLP over c's row features into d_pred (spg_predict.hip).
 int predictor_run(spg_context* c, const spg_predictor* p, float scale, float* d_pred);
 int predictor_device(const spg_predictor* p);
+long long predictor_version(const spg_predictor* p);
 int launch_regroup_intermediate(spg_context* c, int64_t* d_rowptr);
 
 }  // namespace spg

@@ -76,8 +76,8 @@ __device__ __forceinline__ float mlp_forward(const float* P, const float x[kF], 
 __global__ __launch_bounds__(kBlock) void k_mlp_predict(const float4* __restrict__ feat, int m,
                                                          const float* __restrict__ Pg, float4 mu,
                                                          float4 sd, const long long* __restrict__ ub,
-                                                         int n, float scale,
-                                                         float* __restrict__ pred) {
+                                                         int n, const int2* __restrict__ rwin,
+                                                         float scale, float* __restrict__ pred) {
     __shared__ float P[kNP];
     for (int q = threadIdx.x; q < kNP; q += kBlock) P[q] = Pg[q];
     __syncthreads();
@@ -91,8 +91,9 @@ __global__ __launch_bounds__(kBlock) void k_mlp_predict(const float4* __restrict
     float x[kF], h1[kH];
     normalize(feat[i], mu, sd, x);
     const float y = mlp_forward(P, x, h1);
-    float p = expm1f(y + log1p_uniform_count(u, n)) * scale;
-    const float hi = (float)min(u, (long long)n);
+    const int ni = rwin ? rwin[i].y : n;  // columns the row can reach
+    float p = expm1f(y + log1p_uniform_count(u, ni)) * scale;
+    const float hi = (float)min(u, (long long)ni);
     p = p > 0.f ? p : 0.f;  // also maps NaN to 0
     p = p > hi ? hi : p;
     pred[i] = p;
@@ -205,6 +206,7 @@ __global__ void k_mlp_adam(float* __restrict__ P, float* __restrict__ M, float* 
 // rows (count from the symbolic pass, crow = C's row offsets).
 __global__ void k_mlp_samples(const float4* __restrict__ feat, const long long* __restrict__ ub,
                               const long long* __restrict__ crow, int m, int n,
+                              const int2* __restrict__ rwin,
                               const int* __restrict__ pos, float4* __restrict__ sfeat,
                               float* __restrict__ starget) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -212,7 +214,7 @@ __global__ void k_mlp_samples(const float4* __restrict__ feat, const long long* 
     const int q = pos[i];
     sfeat[q] = feat[i];
     const float cnt = (float)(crow[i + 1] - crow[i]);
-    starget[q] = log1pf(cnt) - log1p_uniform_count(ub[i], n);
+    starget[q] = log1pf(cnt) - log1p_uniform_count(ub[i], rwin ? rwin[i].y : n);
 }
 
 __global__ void k_nonempty_flag(const long long* __restrict__ ub, int m, int* __restrict__ flag) {
@@ -235,6 +237,7 @@ struct spg_predictor {
     DevBuf sfeat, starget;        // training samples
     long long nsamples = 0;
     long long adam_t = 0;
+    long long version = 0;  // bumped whenever the weights change
     std::vector<float> host_feat;  // sample features (for the normalization)
     std::string err;
 };
@@ -296,11 +299,12 @@ int predictor_run(spg_context* c, const spg_predictor* p, float scale, float* d_
     const float4 sd = make_float4(p->sd[0], p->sd[1], p->sd[2], p->sd[3]);
     hipLaunchKernelGGL(k_mlp_predict, dim3((c->m + kBlock - 1) / kBlock), dim3(kBlock), 0,
                        c->stream, (const float4*)c->feat.p, c->m, (const float*)p->P.p, mu, sd,
-                       (const long long*)c->ub.p, c->n, scale, d_pred);
+                       (const long long*)c->ub.p, c->n, (const int2*)c->rwin, scale, d_pred);
     SPG_HIP(hipGetLastError());
     return SPG_OK;
 }
 int predictor_device(const spg_predictor* p) { return p->device; }
+long long predictor_version(const spg_predictor* p) { return p->version; }
 }  // namespace spg
 
 extern "C" {
@@ -383,7 +387,7 @@ int spgPredictorAddSamples(spg_predictor* p, spg_context* c) {
     if (rc != SPG_OK) return rc;
     hipLaunchKernelGGL(k_mlp_samples, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream,
                        (const float4*)c->feat.p, (const long long*)c->ub.p,
-                       (const long long*)c->crow.p, m, c->n, pos, (float4*)p->sfeat.p + n0,
+                       (const long long*)c->crow.p, m, c->n, (const int2*)c->rwin, pos, (float4*)p->sfeat.p + n0,
                        (float*)p->starget.p + n0);
     PHIP(hipGetLastError());
     std::vector<float> hf((size_t)added * 4);
@@ -415,6 +419,7 @@ void normalize_once(spg_predictor* p) {
         p->sd[f] = (float)(var > 1e-12 ? std::sqrt(var) : 1.0);
     }
     p->normalized = true;
+    ++p->version;
 }
 }  // namespace
 
@@ -536,6 +541,7 @@ int spgPredictorTrain(spg_predictor* p, int epochs, int batch, float lr, float* 
         return pfail(p, SPG_EHIP, "training kernels failed");
     }
     cleanup();
+    ++p->version;
     if (final_loss) *final_loss = (float)last;
     return SPG_OK;
 }
@@ -557,6 +563,7 @@ int spgPredictorSetWeights(spg_predictor* p, const float* w, const float* norm8)
     if (!p || !w || !norm8) return SPG_EINVAL;
     PHIP(hipSetDevice(p->device));
     PHIP(hipMemcpy(p->P.p, w, sizeof(float) * kNP, hipMemcpyHostToDevice));
+    ++p->version;
     for (int f = 0; f < 4; ++f) {
         p->mu[f] = norm8[f];
         p->sd[f] = norm8[4 + f] > 0.f ? norm8[4 + f] : 1.f;
