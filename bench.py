@@ -48,7 +48,26 @@ WORKLOADS = {
     "uniform_4096x8": dict(kind="uniform", n=4096, k=8, seed=1),
     # C = A*B, A = 2^20 x 2^18 R-MAT (8 * 2^20 draws), B = R-MAT scale 18, ef 8
     "rect_rmat_20x18_ef8": dict(kind="rect", seed=4),
+    # 256 independent C = A*A, n log-uniform in [1024, 65536]; even: uniform
+    # rows with 4..32 nnz per row, odd: R-MAT scale ceil(log2 n) clipped to
+    # n, edge factor 4..16
+    "batch256": dict(kind="batch", count=256, seed=5),
 }
+
+
+def make_batch(count, seed):
+    rng = np.random.default_rng(seed)
+    pairs = []
+    for j in range(count):
+        n = int(np.floor(np.exp(rng.uniform(np.log(1024), np.log(65536)))))
+        if j % 2 == 0:
+            A = spg.gen_uniform_rows(n, n, int(rng.integers(4, 33)), seed * 1000 + j)
+        else:
+            s = int(np.ceil(np.log2(n)))
+            A = spg.gen_rmat(s, s, int(rng.integers(4, 17)) * n, seed=seed * 1000 + j,
+                             clip_rows=n, clip_cols=n)
+        pairs.append((A, A))
+    return pairs
 
 
 def make_workload(name):
@@ -153,6 +172,165 @@ def cpu_baseline(A, B, target_products, threads):
                       f"(upper bound, k-bins, expansion, merge, compaction), {tot_s:.1f} s"}
 
 
+def cpu_baseline_batch(pairs, target_products, threads):
+    """Times the oracle's restated method on whole products of the batch
+    (seeded random order) until target_products."""
+    from oracle import oracle as O
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import csc_of, csr_of
+    order = np.random.default_rng(1).permutation(len(pairs))
+    tot_prod, tot_s, used = 0, 0.0, []
+    for b in order:
+        A, B = pairs[b]
+        m, K, n = A.num_rows, A.num_cols, B.num_cols
+        Acsc = csc_of((A.rows, A.cols, A.vals), (m, K))
+        Bcsr = csr_of((B.rows, B.cols, B.vals), (K, n))
+        t0 = time.perf_counter()
+        r = O.cpu_baseline(m, n, K, Acsc, Bcsr, 0, m, threads)
+        tot_s += time.perf_counter() - t0
+        tot_prod += r["nprod"]
+        used.append(int(b))
+        if tot_prod >= target_products:
+            break
+    return {"value": round(2.0 * tot_prod / tot_s / 1e9, 4), "unit": "GFLOP/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{len(used)} of {len(pairs)} products of the batch (seeded order), "
+                      f"{tot_prod} intermediate products, restated method (upper bound, k-bins, "
+                      f"expansion, merge, compaction), {tot_s:.1f} s"}
+
+
+def bench_batch(args):
+    """256 uneven products: one product per launch sequence (plain path,
+    every product's operands resident on its own context) against the whole
+    batch in one replayed graph (plain, and with learned allocation)."""
+    from spgemm_amd import train_predictor as tp
+    w = WORKLOADS[args.workload]
+    pairs = make_batch(w["count"], w["seed"])
+    if args.batch_only:
+        return bench_batch_only(args, pairs)
+    nprods = []
+    ctxs = []
+    cfg = dict(expand_mode=spg.EXPAND_KBINS)
+    for A, B in pairs:  # operands resident before timing; buffers sized by a warm-up
+        c = spg.Context(0)
+        c.set_config(**cfg)
+        c.set_operands_from_coo(A, B)
+        ctxs.append(c)
+    for _ in range(max(1, args.warmup)):
+        for c in ctxs:
+            c.multiply()
+    ctxs[0].synchronize()
+    nnz_c = []
+    for c in ctxs:
+        nnz_c.append(int(c.get_c_device()[3]))
+    lat = np.zeros((args.steps, len(ctxs)))
+    for s in range(args.steps):
+        for b, c in enumerate(ctxs):
+            t0 = time.perf_counter()
+            nprods_b = c.init_gemm()
+            c.categorize_gemm()
+            c.bin_gemm()
+            c.symbolic_gemm()
+            c.merge_gemm()
+            c.synchronize()
+            lat[s, b] = (time.perf_counter() - t0) * 1e3
+            if s == 0:
+                nprods.append(nprods_b)
+    for c in ctxs:
+        c.close()
+    nprod = int(sum(nprods))
+    flops = 2.0 * nprod
+    plain_ms = float(lat.sum(axis=1).mean())
+    per = lat.mean(axis=0)
+
+    ctx = spg.Context(0)
+    ctx.set_config(**cfg)
+    P, train_info = (tp.train(ctx) if args.retrain else tp.load_or_train(ctx))
+    ctx.batch_set_operands(pairs)
+    res = {}
+    for name, pred in (("batched_plain", None), ("batched_learned", P)):
+        ctx.batch_multiply(predictor=pred)          # eager run: phase times
+        ctx.synchronize()
+        phase = {k: v for k, v in ctx.phase_times().items() if v > 0}
+        st = ctx.batch_multiply(predictor=pred, graph=True)  # record + capture
+        ctx.synchronize()
+        for _ in range(args.warmup):
+            ctx.batch_multiply(predictor=pred, graph=True)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.batch_multiply(predictor=pred, graph=True)
+            ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        res[name] = dict(ms=ms, phase_ms=phase, stats=st)
+    nnz_total = int(ctx.get_c_device()[3])
+    assert nnz_total == sum(nnz_c), "batched and per-product nnz(C) differ"
+    P.close()
+    ctx.close()
+
+    nnzA = sum(A.nnz for A, _ in pairs)
+    nnzB = sum(B.nnz for _, B in pairs)
+    m_tot = sum(A.num_rows for A, _ in pairs)
+    bytes_alg = 16 * nnzA + 8 * nnzB + 16 * nprod + 8 * nnz_total + 8 * (m_tot + len(pairs))
+    head = res["batched_learned"]
+    gfl = {k: flops / (r["ms"] * 1e-3) / 1e9 for k, r in res.items()}
+    plain_gfl = flops / (plain_ms * 1e-3) / 1e9
+    out = {"metric": METRIC, "value": round(gfl["batched_learned"], 2), "unit": "GFLOP/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(head["ms"], 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "batch256: 256 independent C=A*A, n log-uniform in "
+                                  "[1024, 65536], half uniform rows (4-32 nnz/row), half R-MAT "
+                                  "(ef 4-16), seed 5",
+                      "path": "learned allocation + batched (block-diagonal stacking, one "
+                              "replayed HIP graph for the batch)",
+                      "products": len(pairs), "nnz_A": int(nnzA), "nprod": nprod,
+                      "nnz_C": nnz_total, "parallelism": "single GPU"},
+           "phase_ms": {k: round(v, 4) for k, v in head["phase_ms"].items()},
+           "roofline": roofline_of(bytes_alg, head["phase_ms"], args.workload),
+           "plain_one_per_launch": {
+               "value": round(plain_gfl, 2), "ms_per_batch": round(plain_ms, 4),
+               "latency_ms_p50": round(float(np.percentile(per, 50)), 4),
+               "latency_ms_p99": round(float(np.percentile(per, 99)), 4),
+               "nprod_p50": int(np.percentile(nprods, 50)), "nprod_p99": int(np.percentile(nprods, 99))},
+           "learned_batched_vs_plain": round(gfl["batched_learned"] / plain_gfl, 4)}
+    for k, r in res.items():
+        out[k] = {"value": round(gfl[k], 2), "ms_per_batch": round(r["ms"], 4),
+                  "latency_ms_every_product": round(r["ms"], 4),
+                  "eager_phase_ms": {p: round(v, 4) for p, v in r["phase_ms"].items()},
+                  "roofline_frac": roofline_of(bytes_alg, r["phase_ms"], args.workload)["frac"]}
+    st = res["batched_learned"]["stats"]
+    out["batched_learned"]["rates"] = {
+        "overflow_vrow_rate": round(st["overflow_vrows"] / max(st["vrows"], 1), 6),
+        "fallback_slot_fraction": round(st["fallback_slots"] / max(st["nnz"], 1), 6),
+        "padded_slots_per_nnz": round(st["padded_slots"] / max(st["nnz"], 1), 4),
+        "unrecovered_overflows": int(st["unrecovered"])}
+    out["batched_learned"]["predictor_training"] = train_info
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline_batch(pairs, int(args.cpu_products), threads)
+    print(json.dumps(out), flush=True)
+
+
+def bench_batch_only(args, pairs):
+    """The batched runs alone (for profiling): eager plain and learned
+    passes, then graph replays of the learned pass."""
+    from spgemm_amd import train_predictor as tp
+    ctx = spg.Context(0)
+    ctx.set_config(expand_mode=spg.EXPAND_KBINS)
+    P, _ = tp.load_or_train(ctx)
+    ctx.batch_set_operands(pairs)
+    for _ in range(args.warmup + args.steps):
+        ctx.batch_multiply()
+        ctx.batch_multiply(predictor=P)
+    for _ in range(args.steps):
+        ctx.batch_multiply(predictor=P, graph=True)
+    ctx.synchronize()
+    print(json.dumps({"batch_only": True, "phase_ms": ctx.phase_times()}), flush=True)
+    P.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +341,8 @@ def main():
     ap.add_argument("--retrain", action="store_true",
                     help="train the predictor instead of loading spgemm_amd/models/rmat_mix32.mlp")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="batch workloads: skip the one-product-per-launch baseline (profiling)")
     ap.add_argument("--cpu-products", type=float, default=3e9)
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
@@ -171,6 +351,8 @@ def main():
               "C row blocks across GPUs but this build runs on a single MI355X.", file=sys.stderr)
         sys.exit(2)
 
+    if WORKLOADS[args.workload]["kind"] == "batch":
+        return bench_batch(args)
     ctx = spg.Context(0)
     P = train_info = None
     if args.path != "plain":

@@ -301,19 +301,47 @@ int launch_coo2cm(spg_context* c, const triplet* d_trip, int nnz, int type, int 
     return scan_i32(c, cnt, ptr, nmajor);
 }
 
+// Grid of a persistent kernel: every workgroup that can be resident.
+static unsigned persistent_grid(spg_context* c, long long nlist, int per_cu) {
+    long long g = (long long)c->num_cu * per_cu;
+    if (g > nlist) g = nlist;
+    return (unsigned)(g > 0 ? g : 1);
+}
+
 // ------------------------------------------------------------------ init
 // src/mm.cu:138-148 accumulates the upper bound of C row i with one atomic
 // per A(i,k).  Here A is first transposed into row lists (the k of every
 // entry of row i), and every row then sums its bounds without atomics: hub
 // rows of a power-law A would otherwise serialize thousands of atomics on
 // one counter (and T times that for the per-tile bounds of long rows).
+// Eight columns of A per wave, eight lanes per column (a short column
+// takes one or two strided passes), and each column longer than kShortSeg
+// walked by the whole wave in turn: short columns neither idle most of a
+// wave nor chain many dependent atomics on one lane, and hub columns do not
+// serialize.
+constexpr int kShortSeg = 64;
+constexpr int kGroup = 8;
+
 __global__ __launch_bounds__(kBlock) void k_row_hist(const int* __restrict__ a_ptr,
                                                       const int* __restrict__ a_idx, int K,
                                                       int* __restrict__ rcnt) {
-    const long long k = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
-    if (k >= K) return;
-    const int e1 = a_ptr[k + 1];
-    for (int e = a_ptr[k] + lane_id(); e < e1; e += 64) atomicAdd(&rcnt[a_idx[e]], 1);
+    const int lane = lane_id(), sub = lane & (kGroup - 1);
+    const long long k = ((long long)blockIdx.x * kBlock + threadIdx.x) / kGroup;
+    int e0 = 0, e1 = 0;
+    if (k < K) {
+        e0 = a_ptr[k];
+        e1 = a_ptr[k + 1];
+    }
+    const bool lng = e1 - e0 > kShortSeg;
+    if (!lng)
+        for (int e = e0 + sub; e < e1; e += kGroup) atomicAdd(&rcnt[a_idx[e]], 1);
+    unsigned long long mask = __ballot(lng && sub == 0);
+    while (mask) {
+        const int l = __ffsll((long long)mask) - 1;
+        mask &= mask - 1;
+        const int b0 = __shfl(e0, l, 64), b1 = __shfl(e1, l, 64);
+        for (int e = b0 + lane; e < b1; e += 64) atomicAdd(&rcnt[a_idx[e]], 1);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_row_fill(const int* __restrict__ a_ptr,
@@ -322,13 +350,32 @@ __global__ __launch_bounds__(kBlock) void k_row_fill(const int* __restrict__ a_p
                                                       int* __restrict__ rcur,
                                                       int* __restrict__ rk,
                                                       float* __restrict__ rv) {
-    const long long k = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
-    if (k >= K) return;
-    const int e1 = a_ptr[k + 1];
-    for (int e = a_ptr[k] + lane_id(); e < e1; e += 64) {
-        const int pos = atomicAdd(&rcur[a_idx[e]], 1);
-        rk[pos] = (int)k;
-        rv[pos] = a_val[e];
+    const int lane = lane_id(), sub = lane & (kGroup - 1);
+    const long long k = ((long long)blockIdx.x * kBlock + threadIdx.x) / kGroup;
+    int e0 = 0, e1 = 0;
+    if (k < K) {
+        e0 = a_ptr[k];
+        e1 = a_ptr[k + 1];
+    }
+    const bool lng = e1 - e0 > kShortSeg;
+    if (!lng) {
+        for (int e = e0 + sub; e < e1; e += kGroup) {
+            const int pos = atomicAdd(&rcur[a_idx[e]], 1);
+            rk[pos] = (int)k;
+            rv[pos] = a_val[e];
+        }
+    }
+    unsigned long long mask = __ballot(lng && sub == 0);
+    while (mask) {
+        const int l = __ffsll((long long)mask) - 1;
+        mask &= mask - 1;
+        const int b0 = __shfl(e0, l, 64), b1 = __shfl(e1, l, 64);
+        const int kk = (int)(((long long)blockIdx.x * kBlock + (threadIdx.x - lane + l)) / kGroup);
+        for (int e = b0 + lane; e < b1; e += 64) {
+            const int pos = atomicAdd(&rcur[a_idx[e]], 1);
+            rk[pos] = kk;
+            rv[pos] = a_val[e];
+        }
     }
 }
 
@@ -349,10 +396,31 @@ __device__ __forceinline__ T wave_max(T x) {
     return x;
 }
 
-// One wave per row: ub_i = sum of nnz(B row k) over the row's k (the flop
-// upper bound), the column tiling of the row, and the predictor's row
-// features: log1p of nnz(A row i), ub_i, max_k nnz(B row k) and the
-// standard deviation of the nnz(B row k).
+// ub_i = sum of nnz(B row k) over the row's k (the flop upper bound), the
+// column tiling of the row, and the predictor's row features: log1p of
+// nnz(A row i), ub_i, max_k nnz(B row k) and the standard deviation of the
+// nnz(B row k).  Eight rows per wave, eight lanes per short row, each long
+// row by the whole wave (as in k_row_hist).
+__device__ __forceinline__ void row_bound_out(long long i, int r, long long s, long long s2,
+                                              int mx, long long long_ub, int T,
+                                              const int2* __restrict__ rwin, int tile_log2,
+                                              long long* __restrict__ ub,
+                                              int* __restrict__ ntile,
+                                              float4* __restrict__ feat) {
+    ub[i] = s;
+    // tiles of the row's column window (batched products: its block)
+    const int Ti = rwin ? (rwin[i].y + (1 << tile_log2) - 1) >> tile_log2 : T;
+    ntile[i] = (Ti > 1 && s > long_ub) ? Ti : 1;
+    double var = 0.0;
+    if (r > 0) {
+        const double mean = (double)s / r;
+        var = (double)s2 / r - mean * mean;
+        if (var < 0.0) var = 0.0;
+    }
+    feat[i] = make_float4(log1pf((float)r), log1pf((float)s), log1pf((float)mx),
+                          log1pf((float)sqrt(var)));
+}
+
 __global__ __launch_bounds__(kBlock) void k_row_bound(const int* __restrict__ rptr,
                                                        const int* __restrict__ rk,
                                                        const int* __restrict__ b_ptr, int m,
@@ -362,35 +430,54 @@ __global__ __launch_bounds__(kBlock) void k_row_bound(const int* __restrict__ rp
                                                        long long* __restrict__ ub,
                                                        int* __restrict__ ntile,
                                                        float4* __restrict__ feat) {
-    const long long i = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
-    if (i >= m) return;
-    long long s = 0, s2 = 0;
-    int mx = 0;
-    const int e0 = rptr[i], e1 = rptr[i + 1];
-    for (int e = e0 + lane_id(); e < e1; e += 64) {
-        const int k = rk[e];
-        const long long b = b_ptr[k + 1] - b_ptr[k];
-        s += b;
-        s2 += b * b;
-        mx = max(mx, (int)b);
+    const int lane = lane_id(), sub = lane & (kGroup - 1);
+    const long long i = ((long long)blockIdx.x * kBlock + threadIdx.x) / kGroup;
+    int e0 = 0, e1 = 0;
+    if (i < m) {
+        e0 = rptr[i];
+        e1 = rptr[i + 1];
     }
-    s = wave_sum(s);
-    s2 = wave_sum(s2);
-    mx = wave_max(mx);
-    if (lane_id() == 0) {
-        ub[i] = s;
-        // tiles of the row's column window (batched products: its block)
-        const int Ti = rwin ? (rwin[i].y + (1 << tile_log2) - 1) >> tile_log2 : T;
-        ntile[i] = (Ti > 1 && s > long_ub) ? Ti : 1;
-        const int r = e1 - e0;
-        double var = 0.0;
-        if (r > 0) {
-            const double mean = (double)s / r;
-            var = (double)s2 / r - mean * mean;
-            if (var < 0.0) var = 0.0;
+    const bool lng = e1 - e0 > kShortSeg;
+    {
+        long long s = 0, s2 = 0;
+        int mx = 0;
+        if (!lng) {
+            for (int e = e0 + sub; e < e1; e += kGroup) {
+                const int k = rk[e];
+                const long long b = b_ptr[k + 1] - b_ptr[k];
+                s += b;
+                s2 += b * b;
+                mx = max(mx, (int)b);
+            }
         }
-        feat[i] = make_float4(log1pf((float)r), log1pf((float)s), log1pf((float)mx),
-                              log1pf((float)sqrt(var)));
+#pragma unroll
+        for (int o = kGroup / 2; o > 0; o >>= 1) {  // within the 8-lane group
+            s += __shfl_xor(s, o, 64);
+            s2 += __shfl_xor(s2, o, 64);
+            mx = max(mx, __shfl_xor(mx, o, 64));
+        }
+        if (i < m && !lng && sub == 0)
+            row_bound_out(i, e1 - e0, s, s2, mx, long_ub, T, rwin, tile_log2, ub, ntile, feat);
+    }
+    unsigned long long mask = __ballot(lng && sub == 0);
+    while (mask) {
+        const int l = __ffsll((long long)mask) - 1;
+        mask &= mask - 1;
+        const int b0 = __shfl(e0, l, 64), b1 = __shfl(e1, l, 64);
+        long long s = 0, s2 = 0;
+        int mx = 0;
+        for (int e = b0 + lane; e < b1; e += 64) {
+            const int k = rk[e];
+            const long long b = b_ptr[k + 1] - b_ptr[k];
+            s += b;
+            s2 += b * b;
+            mx = max(mx, (int)b);
+        }
+        s = wave_sum(s);
+        s2 = wave_sum(s2);
+        mx = wave_max(mx);
+        if (lane == l)
+            row_bound_out(i, e1 - e0, s, s2, mx, long_ub, T, rwin, tile_log2, ub, ntile, feat);
     }
 }
 
@@ -446,52 +533,61 @@ __global__ void k_bsplit(const int* __restrict__ b_ptr, const int* __restrict__ 
 // every wave sums the tile-t pieces of its share of the row's B rows
 // (coalesced reads of the split points, four rows in flight), then the
 // waves' partial sums are added in LDS.
+__global__ void k_tiled_key(int m, const int* __restrict__ vstart, int* __restrict__ key) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) key[i] = vstart[i + 1] - vstart[i] > 1 ? 0 : -1;
+}
+
+// One workgroup per tiled row (rows list[0 .. *count), persistent).
 __global__ __launch_bounds__(kBlock) void k_row_tile_bound(const int* __restrict__ rptr,
                                                             const int* __restrict__ rk,
                                                             const int* __restrict__ bsplit,
                                                             const int* __restrict__ vstart,
-                                                            int m, int T,
-                                                            long long* __restrict__ vub) {
+                                                            const int* __restrict__ list,
+                                                            const long long* __restrict__ count,
+                                                            int T, long long* __restrict__ vub) {
     __shared__ long long part[kBlock / 64][64];
-    const int i = blockIdx.x;
-    const int vs = vstart[i];
-    const int Ti = vstart[i + 1] - vs;  // the row's tiles (T is the split-point stride - 1)
-    if (Ti == 1) return;
     const int wv = threadIdx.x >> 6, lane = lane_id();
     constexpr int NWV = kBlock / 64;
-    const int e0 = rptr[i], e1 = rptr[i + 1];
-    for (int t0 = 0; t0 < Ti; t0 += 64) {
-        const int t = t0 + lane;
-        long long s = 0;
-        if (t < Ti) {
-            int e = e0 + wv;
-            for (; e + 3 * NWV < e1; e += 4 * NWV) {
-                int kk[4];
+    const long long nr = *count;
+    for (long long q = blockIdx.x; q < nr; q += gridDim.x) {
+        const int i = list[q];
+        const int vs = vstart[i];
+        const int Ti = vstart[i + 1] - vs;  // the row's tiles (T + 1: split-point stride)
+        const int e0 = rptr[i], e1 = rptr[i + 1];
+        for (int t0 = 0; t0 < Ti; t0 += 64) {
+            const int t = t0 + lane;
+            long long s = 0;
+            if (t < Ti) {
+                int e = e0 + wv;
+                for (; e + 3 * NWV < e1; e += 4 * NWV) {
+                    int kk[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) kk[u] = rk[e + u * NWV];
-                int d[4];
+                    for (int u = 0; u < 4; ++u) kk[u] = rk[e + u * NWV];
+                    int d[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int* bs = bsplit + (long long)kk[u] * (T + 1);
-                    d[u] = bs[t + 1] - bs[t];
+                    for (int u = 0; u < 4; ++u) {
+                        const int* bs = bsplit + (long long)kk[u] * (T + 1);
+                        d[u] = bs[t + 1] - bs[t];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) s += d[u];
                 }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) s += d[u];
+                for (; e < e1; e += NWV) {
+                    const int* bs = bsplit + (long long)rk[e] * (T + 1);
+                    s += bs[t + 1] - bs[t];
+                }
             }
-            for (; e < e1; e += NWV) {
-                const int* bs = bsplit + (long long)rk[e] * (T + 1);
-                s += bs[t + 1] - bs[t];
-            }
-        }
-        part[wv][lane] = s;
-        __syncthreads();
-        if (wv == 0 && t < Ti) {
-            long long tot = 0;
+            part[wv][lane] = s;
+            __syncthreads();
+            if (wv == 0 && t < Ti) {
+                long long tot = 0;
 #pragma unroll
-            for (int q = 0; q < NWV; ++q) tot += part[q][lane];
-            vub[vs + t] = tot;
+                for (int u = 0; u < NWV; ++u) tot += part[u][lane];
+                vub[vs + t] = tot;
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
@@ -502,7 +598,7 @@ int launch_init(spg_context* c) {
     int* rcnt = (int*)scratch_alloc(c, sizeof(int) * ((size_t)m + 1));
     if (!rcnt) return fail(c, SPG_ENOMEM, "init scratch");
     SPG_TRY(fill<int>(c, rcnt, 0, m + 1));
-    const long long nbk = ((long long)K * 64 + kBlock - 1) / kBlock;
+    const long long nbk = ((long long)K * kGroup + kBlock - 1) / kBlock;  // 8 lanes per column
     if (K > 0) {
         hipLaunchKernelGGL(k_row_hist, dim3((unsigned)nbk), dim3(kBlock), 0, c->stream, c->A.ptr,
                            c->A.idx, K, rcnt);
@@ -519,7 +615,7 @@ int launch_init(spg_context* c) {
         SPG_HIP(hipGetLastError());
     }
     if (m > 0) {
-        const long long nbm = ((long long)m * 64 + kBlock - 1) / kBlock;
+        const long long nbm = ((long long)m * kGroup + kBlock - 1) / kBlock;  // 8 lanes per row
         hipLaunchKernelGGL(k_row_bound, dim3((unsigned)nbm), dim3(kBlock), 0, c->stream,
                            (const int*)rptr, (const int*)rk, c->B.ptr, m,
                            (long long)c->cfg.long_row_ub, c->T, (const int2*)c->rwin,
@@ -545,9 +641,16 @@ int launch_init_vrows(spg_context* c) {
                            0, c->stream, c->B.ptr, c->B.idx, K, c->T, (const int2*)c->kwin,
                            c->cfg.tile_log2, (int*)c->bsplit.p);
         SPG_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_row_tile_bound, dim3((unsigned)m), dim3(kBlock), 0, c->stream,
-                           (const int*)c->arow_ptr.p, (const int*)c->arow_k.p,
-                           (const int*)c->bsplit.p, (const int*)c->vstart.p, m, c->T,
+        // the tiled rows, compacted on the device (their count stays there)
+        hipLaunchKernelGGL(k_tiled_key, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                           c->stream, m, (const int*)c->vstart.p, (int*)c->vclass.p);
+        SPG_HIP(hipGetLastError());
+        SPG_TRY(stable_partition(c, (const int32_t*)c->vclass.p, m, 1, (int32_t*)c->vlist.p,
+                                 (int64_t*)c->vclass_off.p));
+        hipLaunchKernelGGL(k_row_tile_bound, dim3(persistent_grid(c, m, 8)), dim3(kBlock), 0,
+                           c->stream, (const int*)c->arow_ptr.p, (const int*)c->arow_k.p,
+                           (const int*)c->bsplit.p, (const int*)c->vstart.p,
+                           (const int*)c->vlist.p, (const long long*)c->vclass_off.p + 1, c->T,
                            (long long*)c->vub.p);
         SPG_HIP(hipGetLastError());
     }
@@ -638,13 +741,6 @@ __device__ __forceinline__ long long next_item(unsigned long long* ctr, long lon
     if (threadIdx.x == 0) *sj = (long long)atomicAdd(ctr, 1ull);
     __syncthreads();
     return *sj;
-}
-
-// Grid of a persistent kernel: every workgroup that can be resident.
-static unsigned persistent_grid(spg_context* c, long long nlist, int per_cu) {
-    long long g = (long long)c->num_cu * per_cu;
-    if (g > nlist) g = nlist;
-    return (unsigned)(g > 0 ? g : 1);
 }
 
 // -------------------------------------------------------------- expansion
@@ -1755,95 +1851,101 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
 // pass 1 sets the row's column bitmap, a prefix popcount per word gives
 // every column its rank, pass 2 accumulates into acc[rank] in FP64, and the
 // columns are emitted from the bitmap in ascending order (no sort).
+constexpr int kRankChunk = 8;  // rows taken per counter increment
+
 __global__ __launch_bounds__(kPT) void k_merge_rank(
     const int* __restrict__ vlist, long long nlist, unsigned long long* __restrict__ ctr,
-    const long long* __restrict__ vbase, const int* __restrict__ vlo,
-    const int* __restrict__ vwid, const int* __restrict__ icol,
+    const long long* __restrict__ vbase,
+    const int* __restrict__ vlo, const int* __restrict__ vwid, const int* __restrict__ icol,
     const float* __restrict__ ival, const long long* __restrict__ vout,
     int* __restrict__ ccol, float* __restrict__ cval, long long* __restrict__ cnt_out) {
-    constexpr int WPT = kRankMaxWords / kPT;  // 16 words per thread
     __shared__ unsigned bits[kRankMaxWords];
     __shared__ unsigned short pre[kRankMaxWords / 2];  // rank before each word pair
     __shared__ double acc[kHashMaxCount];
     __shared__ int sh[kPT / 64];
     __shared__ long long sj;
-    const int w0 = threadIdx.x * WPT;
     for (int w = threadIdx.x; w < kRankMaxWords; w += kPT) bits[w] = 0u;
+    // rows in chunks of kRankChunk from a global counter (few atomics, and
+    // rows of uneven size still balance)
     for (;;) {
-        const long long j = next_item(ctr, &sj);
-        if (j >= nlist) break;
-        const int v = vlist[j];
-        const long long s0 = vbase[v], s1 = vbase[v + 1];
-        const long long o = vout[v];
-        const int cnt = (int)min((long long)kHashMaxCount, vout[v + 1] - o);
-        const int lo = vlo[v];                  // columns of the row: [lo, lo + width)
-        const int nw = (vwid[v] + 31) >> 5;     // width <= 2^18 (class condition)
-        // trip count uniform across the workgroup (the OR below shuffles)
-        for (long long sb = s0; sb < s1; sb += 4 * kPT) {
-            int c[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const long long q = sb + threadIdx.x + (long long)u * kPT;
-                c[u] = q < s1 ? icol[q] - lo : -1;
+        const long long chunk = next_item(ctr, &sj);
+        const long long j0 = chunk * kRankChunk;
+        if (j0 >= nlist) break;
+        const long long j1 = min(j0 + kRankChunk, nlist);
+        for (long long j = j0; j < j1; ++j) {
+            const int v = vlist[j];
+            const long long s0 = vbase[v], s1 = vbase[v + 1];
+            const long long o = vout[v];
+            const int cnt = (int)min((long long)kHashMaxCount, vout[v + 1] - o);
+            const int lo = vlo[v];                  // columns of the row: [lo, lo + width)
+            const int nw = (vwid[v] + 31) >> 5;     // width <= 2^18 (class condition)
+            // an even number of bitmap words per thread, enough for the window
+            const int wpt = (((nw + kPT - 1) / kPT) + 1) & ~1;
+            const int w0 = threadIdx.x * wpt;
+            const int w1 = min(w0 + wpt, nw);
+            // trip count uniform across the workgroup (the OR below shuffles)
+            for (long long sb = s0; sb < s1; sb += 4 * kPT) {
+                int c[4];
+    #pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const long long q = sb + threadIdx.x + (long long)u * kPT;
+                    c[u] = q < s1 ? icol[q] - lo : -1;
+                }
+    #pragma unroll
+                for (int u = 0; u < 4; ++u) wave_bitmap_or(bits, c[u]);
             }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) wave_bitmap_or(bits, c[u]);
-        }
-        for (int q = threadIdx.x; q < cnt; q += kPT) acc[q] = 0.0;
-        __syncthreads();
-        int mine = 0;
-#pragma unroll
-        for (int q = 0; q < WPT; ++q) mine += (w0 + q < nw) ? __popc(bits[w0 + q]) : 0;
-        int tot;
-        int run = block_exscan_n<int, kPT>(mine, &tot, sh);
-        if (cnt_out && threadIdx.x == 0) cnt_out[v] = tot;
-#pragma unroll
-        for (int q = 0; q < WPT; q += 2) {
-            pre[(w0 + q) >> 1] = (unsigned short)run;
-            run += __popc(bits[w0 + q]) + __popc(bits[w0 + q + 1]);
-        }
-        __syncthreads();
-        for (long long s = s0 + threadIdx.x; s < s1; s += 4 * kPT) {
-            int c[4];
-            float x[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const long long q = s + (long long)u * kPT;
-                c[u] = -1;
-                x[u] = 0.f;
-                if (q < s1) {
-                    c[u] = icol[q] - lo;
-                    x[u] = ival[q];
+            for (int q = threadIdx.x; q < cnt; q += kPT) acc[q] = 0.0;
+            __syncthreads();
+            int mine = 0;
+            for (int w = w0; w < w1; ++w) mine += __popc(bits[w]);
+            int tot;
+            int run = block_exscan_n<int, kPT>(mine, &tot, sh);
+            if (cnt_out && threadIdx.x == 0) cnt_out[v] = tot;
+            for (int w = w0; w < w1; w += 2) {
+                pre[w >> 1] = (unsigned short)run;
+                run += __popc(bits[w]) + __popc(bits[w + 1]);  // bits past nw are zero
+            }
+            __syncthreads();
+            for (long long s = s0 + threadIdx.x; s < s1; s += 4 * kPT) {
+                int c[4];
+                float x[4];
+    #pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const long long q = s + (long long)u * kPT;
+                    c[u] = -1;
+                    x[u] = 0.f;
+                    if (q < s1) {
+                        c[u] = icol[q] - lo;
+                        x[u] = ival[q];
+                    }
+                }
+    #pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (c[u] >= 0) {
+                        const int w = c[u] >> 5;
+                        const int r = pre[w >> 1] + ((w & 1) ? __popc(bits[w - 1]) : 0) +
+                                      __popc(bits[w] & ((1u << (c[u] & 31)) - 1u));
+                        if (r < cnt) unsafeAtomicAdd(&acc[r], (double)x[u]);
+                    }
                 }
             }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (c[u] >= 0) {
-                    const int w = c[u] >> 5;
-                    const int r = pre[w >> 1] + ((w & 1) ? __popc(bits[w - 1]) : 0) +
-                                  __popc(bits[w] & ((1u << (c[u] & 31)) - 1u));
-                    if (r < cnt) unsafeAtomicAdd(&acc[r], (double)x[u]);
+            __syncthreads();
+            for (int w = w0; w < w1; ++w) {
+                unsigned b = bits[w];
+                int r = pre[w >> 1] + ((w & 1) ? __popc(bits[w - 1]) : 0);
+                while (b) {
+                    const int bit = __ffs(b) - 1;
+                    b &= b - 1;
+                    if (r < cnt) {
+                        ccol[o + r] = lo + (w << 5) + bit;
+                        cval[o + r] = (float)acc[r];
+                    }
+                    ++r;
                 }
             }
+            for (int w = w0; w < w1; ++w) bits[w] = 0u;  // a thread emits and clears its own words
+            __syncthreads();
         }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < WPT; ++q) {
-            const int w = w0 + q;
-            unsigned b = bits[w];
-            int r = pre[w >> 1] + ((w & 1) ? __popc(bits[w - 1]) : 0);
-            while (b) {
-                const int bit = __ffs(b) - 1;
-                b &= b - 1;
-                if (r < cnt) {
-                    ccol[o + r] = lo + (w << 5) + bit;
-                    cval[o + r] = (float)acc[r];
-                }
-                ++r;
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < WPT; ++q) bits[w0 + q] = 0u;
     }
 }
 
@@ -1868,7 +1970,6 @@ static void launch_merge_hash(spg_context* c, const int* list, long long cnt, co
 // tile rows build their column bitmap themselves (no symbolic pass ran).
 static int launch_merge_class(spg_context* c, int b, const int* list, long long cnt,
                               const MergeOut& o, bool build) {
-    unsigned long long* ctr = (unsigned long long*)c->counters.p;
     const long long* vbase = (const long long*)c->vbase.p;
     const int* vlo = (const int*)c->vlo.p;
     const int* icol = (const int*)c->icol.p;
@@ -1896,9 +1997,11 @@ static int launch_merge_class(spg_context* c, int b, const int* list, long long 
                            c->stream, list, cnt, vbase, vlo, (const int*)c->vwid.p, icol, ival,
                            o.obase, o.ocol, o.oval, o.cnt);
     } else if (b == kClassRank) {
-        hipLaunchKernelGGL(k_merge_rank, dim3(persistent_grid(c, cnt, 2)), dim3(kPT), 0,
-                           c->stream, list, cnt, ctr + 2, vbase, vlo, (const int*)c->vwid.p, icol,
-                           ival, o.obase, o.ocol, o.oval, o.cnt);
+        hipLaunchKernelGGL(k_merge_rank,
+                           dim3(persistent_grid(c, (cnt + kRankChunk - 1) / kRankChunk, 2)),
+                           dim3(kPT), 0, c->stream, list, cnt,
+                           (unsigned long long*)c->counters.p + 2, vbase, vlo,
+                           (const int*)c->vwid.p, icol, ival, o.obase, o.ocol, o.oval, o.cnt);
     } else {
         switch (b + kHashMinLog2) {
             case 6: launch_merge_hash<6>(c, list, cnt, o); break;
