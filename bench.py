@@ -4,7 +4,8 @@ python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rmat_s18_ef16]
                 [--path both|learned|plain]
 
 A step is one pass of the hot path over operands already resident in HBM.
-Two paths are timed on the same product:
+Before each path is timed, the autotuner (spgTune, untimed) picks its
+tunables on the same operands.  Two paths are timed on the same product:
 
   plain:   InitGEMM (upper bound, tiling), CategorizeGEMM (k-bins), BinGEMM
            (binned expansion), SymbolicGEMM (exact count), MergeGEMM
@@ -37,6 +38,7 @@ sys.path.insert(0, ROOT)
 import spgemm_amd as spg  # noqa: E402
 
 METRIC = "SpGEMM GFLOP/s & %HBM peak, learned+batched vs plain, exact structure"
+TUNED_FIELDS = ("tile_log2", "long_row_ub", "item_products", "expand_mode", "heavy_factor")
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
@@ -248,7 +250,10 @@ def bench_batch(args):
     P, train_info = (tp.train(ctx) if args.retrain else tp.load_or_train(ctx))
     ctx.batch_set_operands(pairs)
     res = {}
+    base_cfg = ctx.get_config()
     for name, pred in (("batched_plain", None), ("batched_learned", P)):
+        ctx.set_config(**base_cfg)
+        tuned = None if args.no_tune else ctx.tune(predictor=pred, reps=2, seed=0)[0]
         ctx.batch_multiply(predictor=pred)          # eager run: phase times
         ctx.synchronize()
         phase = {k: v for k, v in ctx.phase_times().items() if v > 0}
@@ -262,7 +267,7 @@ def bench_batch(args):
             ctx.batch_multiply(predictor=pred, graph=True)
             ctx.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / args.steps
-        res[name] = dict(ms=ms, phase_ms=phase, stats=st)
+        res[name] = dict(ms=ms, phase_ms=phase, stats=st, tuned=tuned)
     nnz_total = int(ctx.get_c_device()[3])
     assert nnz_total == sum(nnz_c), "batched and per-product nnz(C) differ"
     P.close()
@@ -299,6 +304,8 @@ def bench_batch(args):
                   "latency_ms_every_product": round(r["ms"], 4),
                   "eager_phase_ms": {p: round(v, 4) for p, v in r["phase_ms"].items()},
                   "roofline_frac": roofline_of(bytes_alg, r["phase_ms"], args.workload)["frac"]}
+        if r["tuned"]:
+            out[k]["tuned_config"] = {f: r["tuned"][f] for f in TUNED_FIELDS}
     st = res["batched_learned"]["stats"]
     out["batched_learned"]["rates"] = {
         "overflow_vrow_rate": round(st["overflow_vrows"] / max(st["vrows"], 1), 6),
@@ -341,6 +348,8 @@ def main():
     ap.add_argument("--retrain", action="store_true",
                     help="train the predictor instead of loading spgemm_amd/models/rmat_mix32.mlp")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (untimed) that picks each path's tunables first")
     ap.add_argument("--batch-only", action="store_true",
                     help="batch workloads: skip the one-product-per-launch baseline (profiling)")
     ap.add_argument("--cpu-products", type=float, default=3e9)
@@ -366,11 +375,18 @@ def main():
     m, n = A.num_rows, B.num_cols
 
     res = {}
+    tuned = {}
+    base_cfg = ctx.get_config()
     if args.path != "learned":
+        if not args.no_tune:
+            tuned["plain"] = ctx.tune(reps=2, seed=0)[0]
         (nprod, nnz), ms, ph = timed(ctx, lambda: run_step(ctx), args.steps, args.warmup)
         exact = np.diff(ctx.get_c()[0]) if args.path == "both" else None
         res["plain"] = dict(nprod=nprod, nnz=nnz, ms=ms, phase_ms=ph)
     if args.path != "plain":
+        ctx.set_config(**base_cfg)
+        if not args.no_tune:
+            tuned["learned"] = ctx.tune(predictor=P, reps=2, seed=0)[0]
         (nprod, nnz, st), ms, ph = timed(ctx, lambda: run_step_learned(ctx, P, m), args.steps,
                                          args.warmup)
         res["learned"] = dict(nprod=nprod, nnz=nnz, ms=ms, phase_ms=ph, stats=st)
@@ -417,6 +433,8 @@ def main():
         out["learned"]["predictor_training"] = train_info
     if len(res) == 2:
         out["learned_vs_plain"] = round(gflops["learned"] / gflops["plain"], 4)
+    for k, cfg in tuned.items():
+        out[k]["tuned_config"] = {f: cfg[f] for f in TUNED_FIELDS}
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(A, B, int(args.cpu_products), threads)

@@ -177,6 +177,9 @@ typedef struct spg_config {
     int32_t pred_tight;        /* learned path: 0 = capacity is the merge class's
                                   maximum (fewest overflows), 1 = capacity is the
                                   inflated prediction (least memory)            */
+    int32_t heavy_factor;      /* k is heavy (bin 0) when a_len*b_len >
+                                  heavy_factor * floor(nprod / K); the
+                                  reference's constant is 50 (src/mm.cu:205)   */
 } spg_config;
 
 /* Expansion mappings of BinGEMM.  KBINS groups the (i,k) pairs by k as the
@@ -329,6 +332,21 @@ int spgBatchMultiply(spg_context* ctx, const spg_predictor* p, int flags,
                      spg_learned_stats* st);
 /* Host copy of product b's C (columns relative to its block). */
 int spgBatchGetC(spg_context* ctx, int b, spg_csr64* out);
+
+/* ---------------- autotuner (no reference counterpart) ------------------
+ * Coordinate descent over tile_log2, long_row_ub, item_products,
+ * expand_mode and heavy_factor on the operands set on ctx (one product or a
+ * batch; the learned path when p != NULL): each candidate is timed as the
+ * median wall time of reps full runs, a candidate replaces the incumbent
+ * when it is more than 1 % faster, and seed fixes the order in which the
+ * tunables are visited (two sweeps).  The winner is set on ctx and, if
+ * table_path is not NULL, appended to that table as one JSON line keyed by
+ * the workload's shape.  best / best_ms may be NULL. */
+int spgTune(spg_context* ctx, const spg_predictor* p, int reps, uint64_t seed,
+            const char* table_path, spg_config* best, double* best_ms);
+/* Applies the table's last entry for the shape of the operands on ctx (and
+ * the path: learned != 0), if any; *found (may be NULL) tells. */
+int spgTuneLookup(spg_context* ctx, const char* table_path, int learned, int* found);
 
 #ifdef __cplusplus
 }

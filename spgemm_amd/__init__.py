@@ -69,7 +69,8 @@ class CSR64(Structure):
 class Config(Structure):
     _fields_ = [("tile_log2", c_int32), ("item_products", c_int32), ("long_row_ub", c_int64),
                 ("xcd_swizzle", c_int32), ("timing", c_int32), ("expand_mode", c_int32),
-                ("pred_margin", c_float), ("pred_slack", c_int32), ("pred_tight", c_int32)]
+                ("pred_margin", c_float), ("pred_slack", c_int32), ("pred_tight", c_int32),
+                ("heavy_factor", c_int32)]
 
 
 class LearnedStats(Structure):
@@ -154,6 +155,9 @@ _SIGS = {
     "spgBatchGetLayout": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64)]),
     "spgBatchMultiply": (c_int, [c_void_p, c_void_p, c_int, POINTER(LearnedStats)]),
     "spgBatchGetC": (c_int, [c_void_p, c_int, POINTER(CSR64)]),
+    "spgTune": (c_int, [c_void_p, c_void_p, c_int, c_uint64, c_char_p, POINTER(Config),
+                        POINTER(c_double)]),
+    "spgTuneLookup": (c_int, [c_void_p, c_char_p, c_int, POINTER(c_int)]),
 }
 
 
@@ -510,6 +514,22 @@ class Context:
         val = np.ctypeslib.as_array(out.val, shape=(max(nnz, 1),))[:nnz].copy()
         lib().spgFreeCSR64(byref(out))
         return ptr, idx, val
+
+    # autotuner
+    def tune(self, predictor=None, reps=3, seed=0, table=None):
+        """Picks the fastest tunables for the operands on this context (and
+        keeps them set); returns (config dict, median ms)."""
+        cfg = Config()
+        ms = c_double(0.0)
+        self._c(lib().spgTune(self.h, predictor.h if predictor else None, reps, seed,
+                              table.encode() if table else None, byref(cfg), byref(ms)), "spgTune")
+        return {f: getattr(cfg, f) for f, _ in Config._fields_}, ms.value
+
+    def tune_lookup(self, table, learned=False):
+        found = c_int(0)
+        self._c(lib().spgTuneLookup(self.h, table.encode(), int(learned), byref(found)),
+                "spgTuneLookup")
+        return bool(found.value)
 
     def multiply_learned(self, predictor):
         st = LearnedStats()

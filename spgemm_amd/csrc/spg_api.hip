@@ -164,6 +164,7 @@ int spgCreate(spg_context** out, int device) {
     c->cfg.expand_mode = SPG_EXPAND_KBINS;
     c->cfg.pred_margin = 0.25f;
     c->cfg.pred_slack = 8;
+    c->cfg.heavy_factor = 50;
     int rc = SPG_OK;
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -246,6 +247,8 @@ int spgSetConfig(spg_context* c, const spg_config* cfg) {
         return fail(c, SPG_EINVAL, "pred_slack must be in [0, 2^20]");
     if (cfg->pred_tight != 0 && cfg->pred_tight != 1)
         return fail(c, SPG_EINVAL, "pred_tight must be 0 or 1");
+    if (cfg->heavy_factor < 1 || cfg->heavy_factor > (1 << 20))
+        return fail(c, SPG_EINVAL, "heavy_factor must be in [1, 2^20]");
     c->cfg = *cfg;
     c->stage = std::min(c->stage, 0);
     graph_release(c);

@@ -662,12 +662,12 @@ int launch_init_vrows(spg_context* c) {
 // src/mm.cu:196-257 with D5 (one range), D6 (b_len >= 128 -> bin 8) and D7
 // (64-bit threshold) fixed.
 __global__ void k_kbin(const int* __restrict__ a_ptr, const int* __restrict__ b_ptr, int K,
-                       long long nprod, int* __restrict__ kbin) {
+                       long long nprod, int heavy_factor, int* __restrict__ kbin) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
     const long long alen = a_ptr[k + 1] - a_ptr[k];
     const long long blen = b_ptr[k + 1] - b_ptr[k];
-    const long long thresh = (nprod / K) * 50;
+    const long long thresh = (nprod / K) * heavy_factor;  // the reference's factor: 50
     int bin;
     if (blen <= 0) bin = -1;
     else if (alen * blen > thresh) bin = 0;
@@ -708,7 +708,8 @@ int launch_categorize(spg_context* c) {
     const int K = c->K;
     if (K > 0) {
         hipLaunchKernelGGL(k_kbin, dim3((K + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream,
-                           c->A.ptr, c->B.ptr, K, (long long)c->nprod, (int*)c->kbin.p);
+                           c->A.ptr, c->B.ptr, K, (long long)c->nprod, c->cfg.heavy_factor,
+                           (int*)c->kbin.p);
         SPG_HIP(hipGetLastError());
     }
     SPG_TRY(stable_partition(c, (const int32_t*)c->kbin.p, K, SPG_NUM_KBINS,
