@@ -13,7 +13,7 @@ tunables on the same operands.  Two paths are timed on the same product:
            prediction), LearnedMergeGEMM (merge into predicted capacities,
            exact fallback for overflowed rows, compaction into CSR)
 
-The predictor (spgemm_amd/models/rmat_mix32.mlp, or trained first with
+The predictor (spgemm_amd/models/mix32.mlp, or trained first with
 --retrain, not timed) was trained on the exact counts of 32 seeded smaller
 R-MAT products.  Prints one JSON line; value = the learned path's
 GFLOP/s (flops = 2 * intermediate products), with the plain path beside it.
@@ -346,7 +346,7 @@ def main():
     ap.add_argument("--workload", default="rmat_s18_ef16", choices=sorted(WORKLOADS))
     ap.add_argument("--path", default="both", choices=("both", "learned", "plain"))
     ap.add_argument("--retrain", action="store_true",
-                    help="train the predictor instead of loading spgemm_amd/models/rmat_mix32.mlp")
+                    help="train the predictor instead of loading spgemm_amd/models/mix32.mlp")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (untimed) that picks each path's tunables first")

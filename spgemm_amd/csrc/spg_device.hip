@@ -2088,7 +2088,9 @@ __global__ void k_learned_rowest(int m, const int* __restrict__ vstart,
 }
 
 // Class and output capacity of every virtual row.  The class is chosen from
-// the row's predicted count (inflated by margin and slack); the capacity is
+// the row's predicted count (inflated by margin and slack for the sizes of
+// the tile and hash classes; the raw prediction decides rank vs hash, as the
+// exact count does in the plain merge); the capacity is
 // the most that class's kernel can hold (256 or 1024 columns for the wave
 // tile kernels, the hash table's half for hash rows) or the upper bound
 // min(products, width) for the dense tile and rank kernels, so a row
@@ -2123,6 +2125,7 @@ __global__ void k_learned_class(long long V, const int* __restrict__ vrow_row,
             double want = ceil(share * (1.0 + (double)margin)) + (double)slack;
             want = want < 1.0 ? 1.0 : (want > (double)bound ? (double)bound : want);
             const long long wl = (long long)want;
+            const bool rank_ok = share > (double)kRankMinCount;  // the raw prediction
             if (width <= W) {
                 if (wl <= 256) {
                     k = kClassTileS;
@@ -2134,7 +2137,7 @@ __global__ void k_learned_class(long long V, const int* __restrict__ vrow_row,
                     k = kClassTile;
                     cap = bound;
                 }
-            } else if (wl > kRankMinCount && width <= kRankMaxWords * 32) {
+            } else if (rank_ok && width <= kRankMaxWords * 32) {
                 k = kClassRank;
                 cap = bound;  // <= 4096 = the rank kernel's accumulator
             } else {

@@ -3,10 +3,11 @@
 python -m spgemm_amd.train_predictor [--out spgemm_amd/models/rmat_mix32.mlp]
 
 The training set is the exact row counts (from the symbolic pass) of 32
-seeded R-MAT products: scales 12-15, edge factors 4-16, (a, b, c) =
-(0.57, 0.19, 0.19); every other product is rectangular (A with 4x the rows
-of B).  Training is deterministic for a given seed, so the saved file can be
-regenerated bit for bit.
+seeded products: 16 R-MAT (scales 12-15, edge factors 4-16, (a, b, c) =
+(0.57, 0.19, 0.19), every other one rectangular with 4x the rows of B),
+8 with uniform random rows (n 4096-32768, 4-32 nnz per row) and 8 27-point
+stencils (grids 10^3 to 24^3).  Training is deterministic for a given seed,
+so the saved file can be regenerated bit for bit.
 """
 import argparse
 import os
@@ -14,18 +15,28 @@ import os
 import spgemm_amd as spg
 
 DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models",
-                             "rmat_mix32.mlp")
+                             "mix32.mlp")
 
 
 def training_set(n_mats=32):
     for j in range(n_mats):
-        s = 12 + j % 4
-        ef = (4, 8, 12, 16)[(j // 4) % 4]
-        if j % 2:
-            A = spg.gen_rmat(s + 2, s, ef << (s + 2), seed=1000 + j)
-            B = spg.gen_rmat(s, s, ef << s, seed=2000 + j)
-        else:
-            A = spg.gen_rmat(s, s, ef << s, seed=1000 + j)
+        kind = j % 4
+        if kind in (0, 1):  # R-MAT, square or rectangular
+            s = 12 + (j // 2) % 4
+            ef = (4, 8, 12, 16)[(j // 8) % 4]
+            if kind:
+                A = spg.gen_rmat(s + 2, s, ef << (s + 2), seed=1000 + j)
+                B = spg.gen_rmat(s, s, ef << s, seed=2000 + j)
+            else:
+                A = spg.gen_rmat(s, s, ef << s, seed=1000 + j)
+                B = A
+        elif kind == 2:  # uniform rows
+            n = 4096 << ((j // 4) % 4)
+            A = spg.gen_uniform_rows(n, n, 4 + 4 * ((j // 4) % 8), 3000 + j)
+            B = A
+        else:  # 27-point stencil
+            L = 10 + 2 * (j // 4)
+            A = spg.gen_stencil27(L, L, L, 4000 + j)
             B = A
         yield A, B
 
