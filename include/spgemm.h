@@ -180,6 +180,9 @@ typedef struct spg_config {
     int32_t heavy_factor;      /* k is heavy (bin 0) when a_len*b_len >
                                   heavy_factor * floor(nprod / K); the
                                   reference's constant is 50 (src/mm.cu:205)   */
+    int32_t rank_min_count;    /* wide rows with more distinct columns than this
+                                  merge through a bitmap of their window (rank),
+                                  smaller ones through an LDS hash               */
 } spg_config;
 
 /* Expansion mappings of BinGEMM.  KBINS groups the (i,k) pairs by k as the

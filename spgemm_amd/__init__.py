@@ -70,7 +70,7 @@ class Config(Structure):
     _fields_ = [("tile_log2", c_int32), ("item_products", c_int32), ("long_row_ub", c_int64),
                 ("xcd_swizzle", c_int32), ("timing", c_int32), ("expand_mode", c_int32),
                 ("pred_margin", c_float), ("pred_slack", c_int32), ("pred_tight", c_int32),
-                ("heavy_factor", c_int32)]
+                ("heavy_factor", c_int32), ("rank_min_count", c_int32)]
 
 
 class LearnedStats(Structure):

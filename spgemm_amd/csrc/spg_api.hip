@@ -165,6 +165,7 @@ int spgCreate(spg_context** out, int device) {
     c->cfg.pred_margin = 0.25f;
     c->cfg.pred_slack = 8;
     c->cfg.heavy_factor = 50;
+    c->cfg.rank_min_count = 128;
     int rc = SPG_OK;
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -249,6 +250,8 @@ int spgSetConfig(spg_context* c, const spg_config* cfg) {
         return fail(c, SPG_EINVAL, "pred_tight must be 0 or 1");
     if (cfg->heavy_factor < 1 || cfg->heavy_factor > (1 << 20))
         return fail(c, SPG_EINVAL, "heavy_factor must be in [1, 2^20]");
+    if (cfg->rank_min_count < 0 || cfg->rank_min_count > kHashMaxCount)
+        return fail(c, SPG_EINVAL, "rank_min_count must be in [0, 4096]");
     c->cfg = *cfg;
     c->stage = std::min(c->stage, 0);
     graph_release(c);

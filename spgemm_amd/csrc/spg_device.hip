@@ -1277,7 +1277,8 @@ constexpr int kClassTileS = kNumHashClasses + 2;  // 10: tile rows, count <= 256
 constexpr int kClassTileM = kNumHashClasses + 3;  // 11: tile rows, count <= 1024 (wave)
 constexpr int kTileWords = (1 << kTileMaxLog2) / 32;  // bitmap words of one tile
 constexpr int kRankMaxWords = 8192;              // bitmap of 2^18 columns
-constexpr int kRankMinCount = 128;               // smaller rows: hash + sort
+// Rows over a full-width window with more than cfg.rank_min_count (default
+// 128) distinct columns take the rank kernel, smaller ones a hash.
 
 static_assert(kNumVClasses >= kClassTileM + 1, "class table too small");
 
@@ -1427,6 +1428,131 @@ __global__ void k_rowptr(const int* __restrict__ vstart, const long long* __rest
     if (i <= m) crow[i] = vout[vstart[i]];
 }
 
+// Small wide rows (at most 2^(LOG2H-1) distinct columns, LOG2H <= 9): one
+// wave per row and a wave-private LDS hash, so many rows share a CU without
+// workgroup barriers.
+constexpr int kHashWaveMaxLog2 = 9;
+constexpr int kHashWaveWPB = 8;   // waves per workgroup (4 for 512 slots)
+
+template <int LOG2H, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_symbolic_hash_wave(
+    const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
+    const int* __restrict__ icol, long long* __restrict__ vcnt) {
+    constexpr int H = 1 << LOG2H;
+    __shared__ int keysall[WPB][H];
+    const int wv = threadIdx.x >> 6, lane = lane_id();
+    int* keys = keysall[wv];
+    const long long stride = (long long)gridDim.x * WPB;
+    for (long long j = (long long)blockIdx.x * WPB + wv; j < nlist; j += stride) {
+        const int v = vlist[j];
+        const long long s1 = vbase[v + 1];
+        for (int q = lane; q < H; q += 64) keys[q] = -1;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        int mine = 0;
+        for (long long s = vbase[v] + lane; s < s1; s += 64) {
+            bool fresh;
+            hash_insert<LOG2H>(keys, icol[s], fresh);
+            mine += fresh;
+        }
+        mine = wave_sum(mine);
+        if (lane == 0) vcnt[v] = mine;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Merge of the same rows: keys and FP64 sums in the wave's hash, the occupied
+// slots compacted, and each key's output position found by counting the
+// smaller keys (at most 256 x 256 comparisons of broadcast LDS reads).
+// cnt_out as in k_merge_hash (-1: the predicted table size was too small).
+template <int LOG2H, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_merge_hash_wave(
+    const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
+    const int* __restrict__ icol, const float* __restrict__ ival,
+    const long long* __restrict__ vout, int* __restrict__ ccol, float* __restrict__ cval,
+    long long* __restrict__ cnt_out) {
+    constexpr int H = 1 << LOG2H, P = H / 2, SPL = H / 64;  // slots per lane
+    static_assert(H >= 64 && H <= 512, "wave hash sizes");
+    __shared__ int keysall[WPB][H];
+    __shared__ double valsall[WPB][H];
+    __shared__ int skall[WPB][P];
+    __shared__ double svall[WPB][P];
+    const int wv = threadIdx.x >> 6, lane = lane_id();
+    int* keys = keysall[wv];
+    double* vals = valsall[wv];
+    int* sk = skall[wv];
+    double* sv = svall[wv];
+    const long long stride = (long long)gridDim.x * WPB;
+    for (long long j = (long long)blockIdx.x * WPB + wv; j < nlist; j += stride) {
+        const int v = vlist[j];
+        const long long s1 = vbase[v + 1];
+#pragma unroll
+        for (int q = 0; q < SPL; ++q) {
+            keys[lane + 64 * q] = -1;
+            vals[lane + 64 * q] = 0.0;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        bool full = false;
+        for (long long s = vbase[v] + lane; s < s1; s += 64) {
+            bool fresh;
+            const int h = hash_insert<LOG2H>(keys, icol[s], fresh);
+            if (h >= 0)
+                unsafeAtomicAdd(&vals[h], (double)ival[s]);
+            else
+                full = true;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        bool occ[SPL];
+        int mine = 0;
+#pragma unroll
+        for (int q = 0; q < SPL; ++q) {
+            occ[q] = keys[lane + 64 * q] != -1;
+            mine += occ[q];
+        }
+        int inc = mine;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += y;
+        }
+        const int total = __shfl(inc, 63, 64);
+        if (__ballot(full) != 0ull || total > P) {  // only with a predicted table size
+            if (lane == 0 && cnt_out) cnt_out[v] = -1;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            continue;
+        }
+        int pos = inc - mine;
+#pragma unroll
+        for (int q = 0; q < SPL; ++q) {
+            if (occ[q]) {
+                sk[pos] = keys[lane + 64 * q];
+                sv[pos] = vals[lane + 64 * q];
+                ++pos;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const long long o = vout[v];
+        const int lim = (int)min((long long)total, vout[v + 1] - o);
+        for (int a = lane; a < total; a += 64) {
+            const int key = sk[a];
+            int r = 0;
+            for (int b = 0; b < total; ++b) r += sk[b] < key;
+            if (r < lim) {
+                ccol[o + r] = key;
+                cval[o + r] = (float)sv[a];
+            }
+        }
+        if (lane == 0 && cnt_out) cnt_out[v] = total;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 template <int LOG2H>
 static void launch_sym_hash(spg_context* c, const int* list, long long cnt) {
     hipLaunchKernelGGL(k_symbolic_hash<LOG2H>, dim3((unsigned)cnt), dim3(kBlock), 0, c->stream,
@@ -1466,11 +1592,28 @@ int launch_symbolic(spg_context* c) {
                                (const int*)c->icol.p, (long long*)c->vcnt.p,
                                (unsigned*)c->tilebits.p);
         } else {
+            const dim3 gw(persistent_grid(c, (cnt + kHashWaveWPB - 1) / kHashWaveWPB, 4));
+            const dim3 bw(64 * kHashWaveWPB);
+            const long long* vb = (const long long*)c->vbase.p;
+            const int* ic = (const int*)c->icol.p;
+            long long* vc = (long long*)c->vcnt.p;
             switch (b + kHashMinLog2) {
-                case 6: launch_sym_hash<6>(c, list, cnt); break;
-                case 7: launch_sym_hash<7>(c, list, cnt); break;
-                case 8: launch_sym_hash<8>(c, list, cnt); break;
-                case 9: launch_sym_hash<9>(c, list, cnt); break;
+                case 6:
+                    hipLaunchKernelGGL((k_symbolic_hash_wave<6, kHashWaveWPB>), gw, bw, 0,
+                                       c->stream, list, cnt, vb, ic, vc);
+                    break;
+                case 7:
+                    hipLaunchKernelGGL((k_symbolic_hash_wave<7, kHashWaveWPB>), gw, bw, 0,
+                                       c->stream, list, cnt, vb, ic, vc);
+                    break;
+                case 8:
+                    hipLaunchKernelGGL((k_symbolic_hash_wave<8, kHashWaveWPB>), gw, bw, 0,
+                                       c->stream, list, cnt, vb, ic, vc);
+                    break;
+                case 9:
+                    hipLaunchKernelGGL((k_symbolic_hash_wave<9, kHashWaveWPB>), gw, bw, 0,
+                                       c->stream, list, cnt, vb, ic, vc);
+                    break;
                 case 10: launch_sym_hash<10>(c, list, cnt); break;
                 case 11: launch_sym_hash<11>(c, list, cnt); break;
                 case 12: launch_sym_hash<12>(c, list, cnt); break;
@@ -1499,7 +1642,7 @@ int launch_rowptr(spg_context* c) {
 // columns emitted in ascending order directly at their final CSR position
 // (the symbolic pass fixed every row's offset).
 __global__ void k_vclass_merge(long long V, const long long* __restrict__ vcnt,
-                               const int* __restrict__ vwid, int W, int n,
+                               const int* __restrict__ vwid, int W, int rank_min,
                                int* __restrict__ cls) {
     long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
@@ -1513,7 +1656,7 @@ __global__ void k_vclass_merge(long long V, const long long* __restrict__ vcnt,
         cls[v] = cnt <= 256 ? kClassTileS : (cnt <= 1024 ? kClassTileM : kClassTile);
         return;
     }
-    if (cnt > kRankMinCount && width <= kRankMaxWords * 32 && cnt <= kHashMaxCount) {
+    if (cnt > rank_min && width <= kRankMaxWords * 32 && cnt <= kHashMaxCount) {
         cls[v] = kClassRank;
         return;
     }
@@ -2004,11 +2147,27 @@ static int launch_merge_class(spg_context* c, int b, const int* list, long long 
                            (unsigned long long*)c->counters.p + 2, vbase, vlo,
                            (const int*)c->vwid.p, icol, ival, o.obase, o.ocol, o.oval, o.cnt);
     } else {
+        const dim3 gw(persistent_grid(c, (cnt + kHashWaveWPB - 1) / kHashWaveWPB, 4));
+        const dim3 bw(64 * kHashWaveWPB);
         switch (b + kHashMinLog2) {
-            case 6: launch_merge_hash<6>(c, list, cnt, o); break;
-            case 7: launch_merge_hash<7>(c, list, cnt, o); break;
-            case 8: launch_merge_hash<8>(c, list, cnt, o); break;
-            case 9: launch_merge_hash<9>(c, list, cnt, o); break;
+            case 6:
+                hipLaunchKernelGGL((k_merge_hash_wave<6, kHashWaveWPB>), gw, bw, 0, c->stream,
+                                   list, cnt, vbase, icol, ival, o.obase, o.ocol, o.oval, o.cnt);
+                break;
+            case 7:
+                hipLaunchKernelGGL((k_merge_hash_wave<7, kHashWaveWPB>), gw, bw, 0, c->stream,
+                                   list, cnt, vbase, icol, ival, o.obase, o.ocol, o.oval, o.cnt);
+                break;
+            case 8:
+                hipLaunchKernelGGL((k_merge_hash_wave<8, kHashWaveWPB>), gw, bw, 0, c->stream,
+                                   list, cnt, vbase, icol, ival, o.obase, o.ocol, o.oval, o.cnt);
+                break;
+            case 9:
+                hipLaunchKernelGGL((k_merge_hash_wave<9, 4>),
+                                   dim3(persistent_grid(c, (cnt + 3) / 4, 4)), dim3(256), 0,
+                                   c->stream, list, cnt, vbase, icol, ival, o.obase, o.ocol,
+                                   o.oval, o.cnt);
+                break;
             case 10: launch_merge_hash<10>(c, list, cnt, o); break;
             case 11: launch_merge_hash<11>(c, list, cnt, o); break;
             case 12: launch_merge_hash<12>(c, list, cnt, o); break;
@@ -2026,7 +2185,8 @@ int launch_merge(spg_context* c) {
     if (V > 0) {
         hipLaunchKernelGGL(k_vclass_merge, dim3((unsigned)((V + kBlock - 1) / kBlock)),
                            dim3(kBlock), 0, c->stream, V, (const long long*)c->vcnt.p,
-                           (const int*)c->vwid.p, c->W, c->n, (int*)c->vclass.p);
+                           (const int*)c->vwid.p, c->W, c->cfg.rank_min_count,
+                           (int*)c->vclass.p);
         SPG_HIP(hipGetLastError());
     }
     SPG_TRY(stable_partition(c, (const int32_t*)c->vclass.p, V, kNumVClasses,
@@ -2102,7 +2262,7 @@ __global__ void k_learned_rowest(int m, const int* __restrict__ vstart,
 __global__ void k_learned_class(long long V, const int* __restrict__ vrow_row,
                                 const int* __restrict__ vstart, const long long* __restrict__ vub,
                                 const int* __restrict__ vwid, const double* __restrict__ rowest,
-                                const float* __restrict__ pred, int W, int n, float margin,
+                                const float* __restrict__ pred, int W, int rank_min, float margin,
                                 int slack, int tight, long long* __restrict__ vcap,
                                 int* __restrict__ cls, unsigned long long* __restrict__ nlive) {
     const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2125,7 +2285,7 @@ __global__ void k_learned_class(long long V, const int* __restrict__ vrow_row,
             double want = ceil(share * (1.0 + (double)margin)) + (double)slack;
             want = want < 1.0 ? 1.0 : (want > (double)bound ? (double)bound : want);
             const long long wl = (long long)want;
-            const bool rank_ok = share > (double)kRankMinCount;  // the raw prediction
+            const bool rank_ok = share > (double)rank_min;  // the raw prediction
             if (width <= W) {
                 if (wl <= 256) {
                     k = kClassTileS;
@@ -2158,7 +2318,7 @@ __global__ void k_learned_class(long long V, const int* __restrict__ vrow_row,
 __global__ void k_learned_check(long long V, const long long* __restrict__ vcnt,
                                 const long long* __restrict__ pbase,
                                 const long long* __restrict__ vub, const int* __restrict__ vwid,
-                                int W, int n, long long* __restrict__ fcap, int* __restrict__ cls,
+                                int W, long long* __restrict__ fcap, int* __restrict__ cls,
                                 unsigned long long* __restrict__ nover) {
     const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     bool over = false;
@@ -2184,7 +2344,11 @@ __global__ void k_learned_check(long long V, const long long* __restrict__ vcnt,
 }
 
 // Moves every virtual row's columns and values from the padded (or the
-// fallback) buffer to its CSR position; one wave per row.
+// fallback) buffer to its CSR position.  A wave takes 64 consecutive
+// virtual rows, whose outputs are one contiguous range of C: each lane loads
+// one row's descriptor, and every output position of the range finds its row
+// by a binary search over the 64 row offsets (shuffles), so stores are
+// coalesced and no per-row latency chain is serialized.
 __global__ __launch_bounds__(kPT) void k_learned_compact(
     long long V, const long long* __restrict__ vcnt, const long long* __restrict__ vout,
     const long long* __restrict__ pbase, const long long* __restrict__ fbase,
@@ -2192,46 +2356,110 @@ __global__ __launch_bounds__(kPT) void k_learned_compact(
     const float* __restrict__ fval, int* __restrict__ ccol, float* __restrict__ cval,
     unsigned long long* __restrict__ unrecovered) {
     const int wv = threadIdx.x >> 6, lane = lane_id();
-    const long long stride = (long long)gridDim.x * (kPT / 64);
-    for (long long v = (long long)blockIdx.x * (kPT / 64) + wv; v < V; v += stride) {
-        long long cnt = vcnt[v];
-        if (cnt <= 0) continue;
-        const long long fb = fbase[v], fcap = fbase[v + 1] - fb;
-        const int* sc;
-        const float* sv;
-        long long lim;
-        if (fcap > 0) {
-            sc = fcol + fb;
-            sv = fval + fb;
-            lim = fcap;
-        } else {
-            const long long pb = pbase[v];
-            sc = pcol + pb;
-            sv = pval + pb;
-            lim = pbase[v + 1] - pb;
+    constexpr int NWV = kPT / 64;
+    const long long ngroups = (V + 63) / 64;
+    for (long long grp = (long long)blockIdx.x * NWV + wv; grp < ngroups;
+         grp += (long long)gridDim.x * NWV) {
+        const long long v = grp * 64 + lane;
+        long long cnt = 0, src = 0, o = 0;
+        int from_fb = 0;
+        if (v < V) {
+            cnt = vcnt[v];
+            o = vout[v];
+            const long long fb = fbase[v], fcap = fbase[v + 1] - fb;
+            long long lim;
+            if (fcap > 0) {
+                from_fb = 1;
+                src = fb;
+                lim = fcap;
+            } else {
+                src = pbase[v];
+                lim = pbase[v + 1] - src;
+            }
+            if (cnt > lim) {  // cannot happen: the fallback capacity is an upper bound
+                atomicAdd(unrecovered, 1ull);
+                cnt = lim;
+            }
+            if (cnt < 0) cnt = 0;
+        } else if (V > 0) {
+            o = vout[V];  // past the end: an empty row at the range's end
         }
-        if (cnt > lim) {  // cannot happen: the fallback capacity is an upper bound
-            if (lane == 0) atomicAdd(unrecovered, 1ull);
-            cnt = lim;
+        // a virtual row holds at most 8192 columns (a tile, or a row with at
+        // most 4096 products), so 64 rows' offsets fit in int
+        const long long o0 = __shfl(o, 0, 64);
+        const int rel = (int)(o - o0);
+        const int rend = rel + (int)cnt;  // end of the row's copied range
+        const int total = __shfl((int)(o - o0 + cnt), 63, 64);
+        if (total >= 64 * 256) {
+            // long rows: one row after the other, the whole wave per row
+            for (int qq = 0; qq < 64; ++qq) {
+                const int cq = __shfl((int)cnt, qq, 64);
+                const long long sq = __shfl(src, qq, 64);
+                const long long oq = o0 + __shfl(rel, qq, 64);
+                const bool fq = __shfl(from_fb, qq, 64) != 0;
+                const int* __restrict__ sc = fq ? fcol + sq : pcol + sq;
+                const float* __restrict__ sv = fq ? fval + sq : pval + sq;
+                for (int e0 = 0; e0 < cq; e0 += 4 * 64) {
+                    int c4[4];
+                    float x4[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int e = e0 + u * 64 + lane;
+                        if (e < cq) {
+                            c4[u] = sc[e];
+                            x4[u] = sv[e];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int e = e0 + u * 64 + lane;
+                        if (e < cq) {
+                            ccol[oq + e] = c4[u];
+                            cval[oq + e] = x4[u];
+                        }
+                    }
+                }
+            }
+            continue;
         }
-        const long long o = vout[v];
-        for (long long q0 = 0; q0 < cnt; q0 += 4 * 64) {  // four loads in flight per lane
-            int cc[4];
-            float xx[4];
+        for (int pb = 0; pb < total; pb += 4 * 64) {
+            int p[4], q[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const long long q = q0 + lane + u * 64;
-                if (q < cnt) {
-                    cc[u] = sc[q];
-                    xx[u] = sv[q];
+                p[u] = pb + u * 64 + lane;
+                q[u] = 0;  // largest lane whose range starts at or before p
+            }
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int cand = q[u] + step;
+                    if (__shfl(rel, cand, 64) <= p[u]) q[u] = cand;
+                }
+            }
+            int cc[4];
+            float xx[4];
+            bool ok[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                // every shuffle runs with all lanes active: a lane that is
+                // inactive in a bpermute does not supply its value
+                const long long s = __shfl(src, q[u], 64) + (p[u] - __shfl(rel, q[u], 64));
+                const bool f = __shfl(from_fb, q[u], 64) != 0;
+                const int end = __shfl(rend, q[u], 64);
+                ok[u] = p[u] < total && p[u] < end;
+                cc[u] = 0;
+                xx[u] = 0.f;
+                if (ok[u]) {
+                    cc[u] = f ? fcol[s] : pcol[s];
+                    xx[u] = f ? fval[s] : pval[s];
                 }
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const long long q = q0 + lane + u * 64;
-                if (q < cnt) {
-                    ccol[o + q] = cc[u];
-                    cval[o + q] = xx[u];
+                if (ok[u]) {
+                    ccol[o0 + p[u]] = cc[u];
+                    cval[o0 + p[u]] = xx[u];
                 }
             }
         }
@@ -2277,7 +2505,8 @@ int launch_learned_merge(spg_context* c, spg_learned_stats* st) {
         hipLaunchKernelGGL(k_learned_class, dim3(gV), dim3(kBlock), 0, c->stream, V,
                            (const int*)c->vrow_row.p, (const int*)c->vstart.p,
                            (const long long*)c->vub.p, (const int*)c->vwid.p,
-                           (const double*)rowest, (const float*)c->pred.p, c->W, c->n,
+                           (const double*)rowest, (const float*)c->pred.p, c->W,
+                           c->cfg.rank_min_count,
                            c->cfg.pred_margin, c->cfg.pred_slack, c->cfg.pred_tight, vcap,
                            (int*)c->vclass.p, ctr + kCtrLive);
         SPG_HIP(hipGetLastError());
@@ -2306,7 +2535,7 @@ int launch_learned_merge(spg_context* c, spg_learned_stats* st) {
     if (V > 0) {
         hipLaunchKernelGGL(k_learned_check, dim3(gV), dim3(kBlock), 0, c->stream, V,
                            (const long long*)vcnt, (const long long*)pbase,
-                           (const long long*)c->vub.p, (const int*)c->vwid.p, c->W, c->n, vcap,
+                           (const long long*)c->vub.p, (const int*)c->vwid.p, c->W, vcap,
                            (int*)c->vclass.p, ctr + kCtrOver);
         SPG_HIP(hipGetLastError());
     }
@@ -2341,7 +2570,8 @@ int launch_learned_merge(spg_context* c, spg_learned_stats* st) {
     if (V > 0) {
         const int* fcol = nover > 0 ? (const int*)c->fcol.p : nullptr;
         const float* fval = nover > 0 ? (const float*)c->fval.p : nullptr;
-        hipLaunchKernelGGL(k_learned_compact, dim3(persistent_grid(c, (V + 7) / 8, 4)), dim3(kPT),
+        hipLaunchKernelGGL(k_learned_compact,
+                           dim3(persistent_grid(c, (V + 64 * 8 - 1) / (64 * 8), 4)), dim3(kPT),
                            0, c->stream, V, (const long long*)vcnt, (const long long*)c->vout.p,
                            (const long long*)pbase, (const long long*)fbase,
                            (const int*)c->pcol.p, (const float*)c->pval.p, fcol, fval,

@@ -3,7 +3,8 @@
 // shapes at compile time, src/mm.cu:17,205,262).
 //
 // Coordinate descent from the context's current settings over tile_log2,
-// long_row_ub, item_products, expand_mode and heavy_factor: every value of
+// long_row_ub, item_products, expand_mode, heavy_factor and rank_min_count:
+// every value of
 // one tunable is timed (wall clock of a full product or batch, median of
 // reps runs after one warm-up) with the others held, the fastest kept if it
 // beats the incumbent by more than 1 %, and the tunables are visited in an
@@ -35,7 +36,8 @@ long long get(const spg_config& c, int d) {
         case 1: return c.long_row_ub;
         case 2: return c.item_products;
         case 3: return c.expand_mode;
-        default: return c.heavy_factor;
+        case 4: return c.heavy_factor;
+        default: return c.rank_min_count;
     }
 }
 
@@ -45,7 +47,8 @@ void set(spg_config& c, int d, long long v) {
         case 1: c.long_row_ub = v; break;
         case 2: c.item_products = (int32_t)v; break;
         case 3: c.expand_mode = (int32_t)v; break;
-        default: c.heavy_factor = (int32_t)v; break;
+        case 4: c.heavy_factor = (int32_t)v; break;
+        default: c.rank_min_count = (int32_t)v; break;
     }
 }
 
@@ -101,15 +104,16 @@ int spgTune(spg_context* c, const spg_predictor* p, int reps, uint64_t seed,
         {"item_products", {2048, 4096, 8192, 16384, 32768}},
         {"expand_mode", {SPG_EXPAND_KBINS, SPG_EXPAND_ROWS}},
         {"heavy_factor", {10, 25, 50, 100, 200}},
+        {"rank_min_count", {64, 128, 256, 512, 1024}},
     };
     const int32_t timing = c->cfg.timing;  // the caller's setting, restored on the winner
     spg_config inc = c->cfg;
     inc.timing = 0;
     double inc_ms = time_config(c, p, inc, reps);
     if (inc_ms < 0) return fail(c, SPG_EINVAL, "the starting configuration does not run");
-    std::vector<int> order = {0, 1, 2, 3, 4};
+    std::vector<int> order = {0, 1, 2, 3, 4, 5};
     Xoshiro rng(seed);
-    for (int q = 4; q > 0; --q) std::swap(order[(size_t)q], order[(size_t)rng.below((uint64_t)q + 1)]);
+    for (int q = 5; q > 0; --q) std::swap(order[(size_t)q], order[(size_t)rng.below((uint64_t)q + 1)]);
     for (int sweep = 0; sweep < 2; ++sweep) {
         for (int d : order) {
             const long long cur = get(inc, d);
@@ -134,9 +138,10 @@ int spgTune(spg_context* c, const spg_predictor* p, int reps, uint64_t seed,
         if (!f) return fail(c, SPG_EIO, "cannot open the tuning table for appending");
         fprintf(f,
                 "{\"key\":\"%s\",\"tile_log2\":%d,\"long_row_ub\":%lld,\"item_products\":%d,"
-                "\"expand_mode\":%d,\"heavy_factor\":%d,\"ms\":%.4f,\"seed\":%llu}\n",
+                "\"expand_mode\":%d,\"heavy_factor\":%d,\"rank_min_count\":%d,\"ms\":%.4f,"
+                "\"seed\":%llu}\n",
                 workload_key(c, p != nullptr).c_str(), inc.tile_log2, (long long)inc.long_row_ub,
-                inc.item_products, inc.expand_mode, inc.heavy_factor, inc_ms,
+                inc.item_products, inc.expand_mode, inc.heavy_factor, inc.rank_min_count, inc_ms,
                 (unsigned long long)seed);
         if (fclose(f) != 0) return fail(c, SPG_EIO, "tuning table write failed");
     }
@@ -163,6 +168,7 @@ int spgTuneLookup(spg_context* c, const char* table_path, int learned, int* foun
     if (field(hit, "item_products", &v)) cfg.item_products = (int32_t)v;
     if (field(hit, "expand_mode", &v)) cfg.expand_mode = (int32_t)v;
     if (field(hit, "heavy_factor", &v)) cfg.heavy_factor = (int32_t)v;
+    if (field(hit, "rank_min_count", &v)) cfg.rank_min_count = (int32_t)v;
     SPG_TRY(spgSetConfig(c, &cfg));
     if (found) *found = 1;
     return SPG_OK;

@@ -134,7 +134,12 @@ def test_batch_learned(spg, orc, gpu_ctx):
             P.add_samples(ctx)
         P.train(epochs=10, batch=512, lr=2e-3)
         ctx.batch_set_operands(pairs)
-        for graph in (False, True, True):
+        # eager; recorded + captured; replayed
+        for graph, scramble in ((False, True), (True, True), (True, False)):
+            if scramble:  # other output in the C buffers (unwritten positions would show)
+                ctx.batch_set_operands(make_batch(spg, 99, count=3))
+                ctx.batch_multiply()
+                ctx.batch_set_operands(pairs)
             st = ctx.batch_multiply(predictor=P, graph=graph)
             ctx.synchronize()
             print(f"  graph={graph} stats {st}")

@@ -53,7 +53,18 @@ def plain(spg, ctx, A, Ash, B, Bsh, cfg=None):
     return ctx.get_c()
 
 
+def scramble(spg, ctx):
+    """Fills the context's C buffers with another product's output, so a
+    position the learned path fails to write cannot pass by holding the
+    plain run's value."""
+    U = spg.gen_uniform_rows(20000, 20000, 24, 12345)
+    ctx.set_config(**BASE_CFG)
+    ctx.set_operands_from_coo(U, U)
+    ctx.multiply()
+
+
 def learned(spg, ctx, A, Ash, B, Bsh, pred=None, predictor=None, cfg=None):
+    scramble(spg, ctx)
     load(spg, ctx, A, Ash, B, Bsh, cfg)
     ctx.init_gemm()
     ctx.categorize_gemm()

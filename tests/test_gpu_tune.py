@@ -10,7 +10,8 @@ pytestmark = pytest.mark.gpu
 
 CANDIDATES = {"tile_log2": {11, 12, 13}, "long_row_ub": {512, 1024, 2048, 4096},
               "item_products": {2048, 4096, 8192, 16384, 32768}, "expand_mode": {0, 1},
-              "heavy_factor": {10, 25, 50, 100, 200}}
+              "heavy_factor": {10, 25, 50, 100, 200},
+              "rank_min_count": {64, 128, 256, 512, 1024}}
 
 
 def test_tune_table_roundtrip(spg, orc, gpu_ctx, tmp_path):
