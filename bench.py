@@ -38,7 +38,8 @@ sys.path.insert(0, ROOT)
 import spgemm_amd as spg  # noqa: E402
 
 METRIC = "SpGEMM GFLOP/s & %HBM peak, learned+batched vs plain, exact structure"
-TUNED_FIELDS = ("tile_log2", "long_row_ub", "item_products", "expand_mode", "heavy_factor")
+TUNED_FIELDS = ("tile_log2", "long_row_ub", "item_products", "expand_mode", "heavy_factor",
+                "rank_min_count")
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
