@@ -791,19 +791,19 @@ struct ExpandLds {
 __device__ void expand_work(const ExpandArgs& g, ExpandLds& L, int k, long long e_begin,
                             long long ne, int j0, int nj, int lg) {
     const int G = 1 << lg;
-    int NE = kBlock >> lg;
+    const int NE = kBlock >> lg;  // entries written at a time (G lanes each)
     const int T = g.T;
     const int b0 = g.b_ptr[k];
     const int tfirst = g.kwin ? (g.kwin[k].x >> g.tile_log2) : 0;  // windows are W-aligned
+    // entries whose slots are reserved per round: as many as the LDS tables
+    // hold, so one round of atomics serves several rounds of writes
+    const int R = T > 1 ? min(kBlock, kTBCap / T) : kBlock;
     __syncthreads();  // LDS of the previous item is free
-    if (T > 1) {
-        const int cap = kTBCap / T;
-        if (cap < NE) NE = cap;
+    if (T > 1)
         for (int t = threadIdx.x; t <= T; t += kBlock) L.sbs[t] = g.bsplit[(long long)k * (T + 1) + t];
-    }
     const int jlo = b0 + j0, jhi = b0 + j0 + nj;
-    for (long long eb = 0; eb < ne; eb += NE) {
-        const int cnt = (int)min((long long)NE, ne - eb);
+    for (long long eb = 0; eb < ne; eb += R) {
+        const int cnt = (int)min((long long)R, ne - eb);
         if (eb > 0) __syncthreads();
         if (threadIdx.x < cnt) {
             const long long e = e_begin + eb + threadIdx.x;
@@ -833,8 +833,9 @@ __device__ void expand_work(const ExpandArgs& g, ExpandLds& L, int k, long long 
             }
             __syncthreads();
         }
-        const int q = threadIdx.x >> lg;
-        if (q < cnt) {
+        for (int sb = 0; sb < cnt; sb += NE) {  // writes, NE entries at a time
+            const int q = sb + (threadIdx.x >> lg);
+            if (q >= cnt) continue;
             const float a = L.sa[q];
             const int fl = L.sflag[q];
             const long long base = L.sbase[q];
