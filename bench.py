@@ -254,7 +254,7 @@ def bench_batch(args):
     base_cfg = ctx.get_config()
     for name, pred in (("batched_plain", None), ("batched_learned", P)):
         ctx.set_config(**base_cfg)
-        tuned = None if args.no_tune else ctx.tune(predictor=pred, reps=2, seed=0)[0]
+        tuned = apply_tuning(ctx, args, pred, pred is not None)
         ctx.batch_multiply(predictor=pred)          # eager run: phase times
         ctx.synchronize()
         phase = {k: v for k, v in ctx.phase_times().items() if v > 0}
@@ -339,6 +339,34 @@ def bench_batch_only(args, pairs):
     ctx.close()
 
 
+def copy_probe(mib):
+    """One device-to-device copy of mib MiB (the HIP runtime's copy kernel):
+    the known-size transfer that scales the FETCH_SIZE / WRITE_SIZE counters
+    of a rocprofv3 --pmc run into bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    for _ in range(2):  # 3 = hipMemcpyDeviceToDevice
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": n, "copies": 2}), flush=True)
+
+
+def apply_tuning(ctx, args, predictor, learned):
+    """Tunes the context for the path (untimed), or applies the entry of
+    --tune-table for this workload when there is one; returns the settings."""
+    if args.no_tune:
+        return None
+    if args.tune_table and ctx.tune_lookup(args.tune_table, learned=learned):
+        return ctx.get_config()
+    return ctx.tune(predictor=predictor, reps=2, seed=0, table=args.tune_table)[0]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -351,11 +379,17 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (untimed) that picks each path's tunables first")
+    ap.add_argument("--tune-table", default=None,
+                    help="JSON-lines table: apply its entry for the workload, else tune and append")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this size (counter calibration)")
     ap.add_argument("--batch-only", action="store_true",
                     help="batch workloads: skip the one-product-per-launch baseline (profiling)")
     ap.add_argument("--cpu-products", type=float, default=3e9)
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
+    if args.copy_probe_mib:
+        return copy_probe(args.copy_probe_mib)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by "
               "C row blocks across GPUs but this build runs on a single MI355X.", file=sys.stderr)
@@ -379,15 +413,13 @@ def main():
     tuned = {}
     base_cfg = ctx.get_config()
     if args.path != "learned":
-        if not args.no_tune:
-            tuned["plain"] = ctx.tune(reps=2, seed=0)[0]
+        tuned["plain"] = apply_tuning(ctx, args, None, False)
         (nprod, nnz), ms, ph = timed(ctx, lambda: run_step(ctx), args.steps, args.warmup)
         exact = np.diff(ctx.get_c()[0]) if args.path == "both" else None
         res["plain"] = dict(nprod=nprod, nnz=nnz, ms=ms, phase_ms=ph)
     if args.path != "plain":
         ctx.set_config(**base_cfg)
-        if not args.no_tune:
-            tuned["learned"] = ctx.tune(predictor=P, reps=2, seed=0)[0]
+        tuned["learned"] = apply_tuning(ctx, args, P, True)
         (nprod, nnz, st), ms, ph = timed(ctx, lambda: run_step_learned(ctx, P, m), args.steps,
                                          args.warmup)
         res["learned"] = dict(nprod=nprod, nnz=nnz, ms=ms, phase_ms=ph, stats=st)
@@ -435,7 +467,8 @@ def main():
     if len(res) == 2:
         out["learned_vs_plain"] = round(gflops["learned"] / gflops["plain"], 4)
     for k, cfg in tuned.items():
-        out[k]["tuned_config"] = {f: cfg[f] for f in TUNED_FIELDS}
+        if cfg:
+            out[k]["tuned_config"] = {f: cfg[f] for f in TUNED_FIELDS}
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(A, B, int(args.cpu_products), threads)
