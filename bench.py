@@ -15,8 +15,10 @@ tunables on the same operands.  Two paths are timed on the same product:
 
 The predictor (spgemm_amd/models/mix32.mlp, or trained first with
 --retrain, not timed) was trained on the exact counts of 32 seeded smaller
-R-MAT products.  Prints one JSON line; value = the learned path's
-GFLOP/s (flops = 2 * intermediate products), with the plain path beside it.
+products (R-MAT, uniform rows, stencils).  Prints one JSON line; value = the
+GFLOP/s of the path the workload is defined on (the R-MAT s18, stencil and
+uniform products: the plain exact-count path; the rectangular product: the
+learned path; batch256: learned + batched), with the other path beside it.
 The roofline object prices the expansion + merge kernels of one step
 against MI355X's 8 TB/s HBM peak with the algorithmic byte count
     16*nnz(A) + 8*nnz(B) + 16*nprod + 8*nnz(C) + 8*(m+1)
@@ -43,14 +45,17 @@ TUNED_FIELDS = ("tile_log2", "long_row_ub", "item_products", "expand_mode", "hea
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
+    # headline: the path the workload is defined on (the other is reported beside)
     # C = A*A, A = R-MAT scale 18, edge factor 16, (0.57, 0.19, 0.19, 0.05)
-    "rmat_s18_ef16": dict(kind="rmat", row_bits=18, col_bits=18, ef=16, seed=0),
+    "rmat_s18_ef16": dict(kind="rmat", row_bits=18, col_bits=18, ef=16, seed=0,
+                          headline="plain"),
     # C = A*A, A = 27-point stencil on 64^3
-    "stencil27_64": dict(kind="stencil", L=64, seed=3),
+    "stencil27_64": dict(kind="stencil", L=64, seed=3, headline="plain"),
     # C = A*A, n = 4096, 8 distinct columns per row
-    "uniform_4096x8": dict(kind="uniform", n=4096, k=8, seed=1),
-    # C = A*B, A = 2^20 x 2^18 R-MAT (8 * 2^20 draws), B = R-MAT scale 18, ef 8
-    "rect_rmat_20x18_ef8": dict(kind="rect", seed=4),
+    "uniform_4096x8": dict(kind="uniform", n=4096, k=8, seed=1, headline="plain"),
+    # C = A*B, A = 2^20 x 2^18 R-MAT (8 * 2^20 draws), B = R-MAT scale 18, ef 8;
+    # learned allocation (mispredict, overflow and fallback rates reported)
+    "rect_rmat_20x18_ef8": dict(kind="rect", seed=4, headline="learned"),
     # 256 independent C = A*A, n log-uniform in [1024, 65536]; even: uniform
     # rows with 4..32 nnz per row, odd: R-MAT scale ceil(log2 n) clipped to
     # n, edge factor 4..16
@@ -121,19 +126,19 @@ def timed(ctx, step, steps, warmup):
     return r, ms, {k: v / steps for k, v in phase_sum.items() if v > 0}
 
 
-def roofline_of(bytes_alg, phase_ms, workload):
+def roofline_of(bytes_alg, phase_ms, workload, path):
     t_em = (phase_ms.get("expand", 0.0) + phase_ms.get("merge", 0.0)) * 1e-3
     achieved = bytes_alg / t_em / 1e9 if t_em > 0 else 0.0
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(workload),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(workload, path),
             "kernels": "expansion + merge kernels of one step (k_expand*, k_merge_*)",
             "bytes_alg_per_step": int(bytes_alg), "kernel_ms_per_step": round(t_em * 1e3, 4)}
 
 
-def load_traffic(workload):
-    """HBM bytes per step of the expansion+merge kernels measured with
-    rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), if a summary exists."""
-    p = os.path.join(ROOT, "profiles", f"pmc_{workload}.json")
+def load_traffic(workload, path):
+    """HBM bytes per step of the expansion+merge kernels of the path measured
+    with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), if a summary exists."""
+    p = os.path.join(ROOT, "profiles", f"pmc_{workload}_{path}.json")
     if not os.path.exists(p):
         return None
     try:
@@ -293,7 +298,7 @@ def bench_batch(args):
                       "products": len(pairs), "nnz_A": int(nnzA), "nprod": nprod,
                       "nnz_C": nnz_total, "parallelism": "single GPU"},
            "phase_ms": {k: round(v, 4) for k, v in head["phase_ms"].items()},
-           "roofline": roofline_of(bytes_alg, head["phase_ms"], args.workload),
+           "roofline": roofline_of(bytes_alg, head["phase_ms"], args.workload, "learned"),
            "plain_one_per_launch": {
                "value": round(plain_gfl, 2), "ms_per_batch": round(plain_ms, 4),
                "latency_ms_p50": round(float(np.percentile(per, 50)), 4),
@@ -304,7 +309,8 @@ def bench_batch(args):
         out[k] = {"value": round(gfl[k], 2), "ms_per_batch": round(r["ms"], 4),
                   "latency_ms_every_product": round(r["ms"], 4),
                   "eager_phase_ms": {p: round(v, 4) for p, v in r["phase_ms"].items()},
-                  "roofline_frac": roofline_of(bytes_alg, r["phase_ms"], args.workload)["frac"]}
+                  "roofline_frac": roofline_of(bytes_alg, r["phase_ms"], args.workload,
+                                               "learned")["frac"]}
         if r["tuned"]:
             out[k]["tuned_config"] = {f: r["tuned"][f] for f in TUNED_FIELDS}
     st = res["batched_learned"]["stats"]
@@ -439,12 +445,14 @@ def main():
             "padded_slots_per_nnz": round(st["padded_slots"] / max(st["nnz"], 1), 4),
             "unrecovered_overflows": int(st["unrecovered"])}
 
-    head = res["learned"] if "learned" in res else res["plain"]
+    path_name = WORKLOADS[args.workload]["headline"]
+    if path_name not in res:
+        path_name = "learned" if "learned" in res else "plain"
+    head = res[path_name]
     nprod, nnz = head["nprod"], head["nnz"]
     flops = 2.0 * nprod
     gflops = {k: flops / (r["ms"] * 1e-3) / 1e9 for k, r in res.items()}
     bytes_alg = 16 * A.nnz + 8 * B.nnz + 16 * nprod + 8 * nnz + 8 * (m + 1)
-    path_name = "learned" if "learned" in res else "plain"
     out = {"metric": METRIC, "value": round(gflops[path_name], 2), "unit": "GFLOP/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(head["ms"], 4),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
@@ -456,11 +464,11 @@ def main():
                       "m": m, "k": A.num_cols, "n": n, "nnz_A": int(A.nnz), "nnz_B": int(B.nnz),
                       "nprod": int(nprod), "nnz_C": int(nnz), "parallelism": "single GPU"},
            "phase_ms": {k: round(v, 4) for k, v in head["phase_ms"].items()},
-           "roofline": roofline_of(bytes_alg, head["phase_ms"], args.workload)}
+           "roofline": roofline_of(bytes_alg, head["phase_ms"], args.workload, path_name)}
     for k, r in res.items():
         out[k] = {"value": round(gflops[k], 2), "ms_per_step": round(r["ms"], 4),
                   "phase_ms": {p: round(v, 4) for p, v in r["phase_ms"].items()},
-                  "roofline_frac": roofline_of(bytes_alg, r["phase_ms"], args.workload)["frac"]}
+                  "roofline_frac": roofline_of(bytes_alg, r["phase_ms"], args.workload, k)["frac"]}
     if "learned" in res:
         out["learned"]["rates"] = res["learned"]["rates"]
         out["learned"]["predictor_training"] = train_info
