@@ -1276,12 +1276,15 @@ constexpr int kClassTile = kNumHashClasses;       // 8: large tile rows (workgro
 constexpr int kClassRank = kNumHashClasses + 1;   // 9
 constexpr int kClassTileS = kNumHashClasses + 2;  // 10: tile rows, count <= 256 (wave)
 constexpr int kClassTileM = kNumHashClasses + 3;  // 11: tile rows, count <= 1024 (wave)
+constexpr int kClassRankW = kNumHashClasses + 4;  // 12: rank rows of a narrow window (wave)
 constexpr int kTileWords = (1 << kTileMaxLog2) / 32;  // bitmap words of one tile
 constexpr int kRankMaxWords = 8192;              // bitmap of 2^18 columns
+constexpr int kRankWaveWords = 2048;             // wave rank kernel: windows <= 2^16 columns
+constexpr int kRankWaveCap = 1024;               // ... and counts <= 1024
 // Rows over a full-width window with more than cfg.rank_min_count (default
 // 128) distinct columns take the rank kernel, smaller ones a hash.
 
-static_assert(kNumVClasses >= kClassTileM + 1, "class table too small");
+static_assert(kNumVClasses >= kClassRankW + 1, "class table too small");
 
 __global__ void k_vclass_symbolic(long long V, const long long* __restrict__ vub,
                                   const int* __restrict__ vwid, int W,
@@ -1373,15 +1376,27 @@ __global__ __launch_bounds__(kPT) void k_symbolic_tile(
     int v = vlist[j];
     long long s0 = vbase[v], s1 = vbase[v + 1];
     int lo = vlo[v];
+    // 16-byte groups past the row's end are clamped to the group holding its
+    // last product (rows of the class are non-empty; the intermediates are
+    // padded to whole groups) and masked where used, so no load sits under a
+    // lane-divergent branch; the next groups are in flight while the
+    // current ones are applied
+    int4 q[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+        q[u] = *(const int4*)(icol + min((s0 & ~3LL) + (long long)(lane + u * 64) * 4,
+                                         (s1 - 1) & ~3LL));
     for (;;) {
         const long long jn = j + stride;
         const int vn = jn < nlist ? vlist[jn] : -1;
-        for (long long sb = s0 & ~3LL; sb < s1; sb += 2 * 4 * 64) {
-            int4 q[2];
+        for (long long sb = s0 & ~3LL;;) {
+            const long long nsb = sb + 2 * 4 * 64;
+            int4 qn[2] = {q[0], q[1]};
+            if (nsb < s1) {  // uniform
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                const long long base = sb + (long long)(lane + u * 64) * 4;
-                q[u] = base < s1 ? *(const int4*)(icol + base) : make_int4(0, 0, 0, 0);
+                for (int u = 0; u < 2; ++u)
+                    qn[u] = *(const int4*)(icol + min(nsb + (long long)(lane + u * 64) * 4,
+                                                      (s1 - 1) & ~3LL));
             }
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
@@ -1395,6 +1410,10 @@ __global__ __launch_bounds__(kPT) void k_symbolic_tile(
                     }
                 }
             }
+            q[0] = qn[0];
+            q[1] = qn[1];
+            sb = nsb;
+            if (sb >= s1) break;
         }
         long long s0n = 0, s1n = 0;
         int lon = 0;
@@ -1402,6 +1421,10 @@ __global__ __launch_bounds__(kPT) void k_symbolic_tile(
             s0n = vbase[vn];
             s1n = vbase[vn + 1];
             lon = vlo[vn];
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+                q[u] = *(const int4*)(icol + min((s0n & ~3LL) + (long long)(lane + u * 64) * 4,
+                                                 (s1n - 1) & ~3LL));
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -1657,6 +1680,10 @@ __global__ void k_vclass_merge(long long V, const long long* __restrict__ vcnt,
         cls[v] = cnt <= 256 ? kClassTileS : (cnt <= 1024 ? kClassTileM : kClassTile);
         return;
     }
+    if (cnt > rank_min && width <= kRankWaveWords * 32 && cnt <= kRankWaveCap) {
+        cls[v] = kClassRankW;
+        return;
+    }
     if (cnt > rank_min && width <= kRankMaxWords * 32 && cnt <= kHashMaxCount) {
         cls[v] = kClassRank;
         return;
@@ -1753,37 +1780,21 @@ __global__ __launch_bounds__(kBlock) void k_merge_hash(
 }
 
 // Tile-width virtual rows: dense FP64 accumulator over the tile's columns
-// plus an occupancy bitmap.  Persistent workgroups walk the list with a
-// fixed stride; the next row's descriptor and first 4*kPT products are
-// loaded before the current row is emitted, so their latency overlaps the
-// emission.  Only touched accumulators are cleared (while being emitted).
-__device__ __forceinline__ void load4(const int* __restrict__ icol,
-                                      const float* __restrict__ ival, long long s,
-                                      long long s1, int lo, int (&c)[4], float (&x)[4]) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const long long q = s + (long long)u * kPT;
-        c[u] = -1;
-        x[u] = 0.f;
-        if (q < s1) {
-            c[u] = icol[q] - lo;
-            x[u] = ival[q];
-        }
-    }
-}
-
-__device__ __forceinline__ void apply4(double* acc, unsigned* bits, const int (&c)[4],
-                                       const float (&x)[4]) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        wave_bitmap_or(bits, c[u]);
-        if (c[u] >= 0) unsafeAtomicAdd(&acc[c[u]], (double)x[u]);
-    }
-}
-
+// plus an occupancy bitmap (stored by the symbolic pass, or built here with
+// BUILD when none ran).  Persistent workgroups walk the list with a fixed
+// stride; each thread loads four consecutive products at a time (16-byte
+// loads) and the next four are in flight while the current ones are
+// accumulated; the next row's descriptor and first products are loaded
+// before the current row is emitted.  The emit walks the columns in order,
+// one per thread (consecutive threads read consecutive accumulators, so no
+// LDS bank is hit twice, and write consecutive output positions); its rank
+// is the word's prefix plus the bits below it.  Only touched accumulators
+// are cleared (while being emitted).
+template <bool BUILD>
 __global__ __launch_bounds__(kPT) void k_merge_tile(
     const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
-    const int* __restrict__ vlo, const int* __restrict__ vwid, const int* __restrict__ icol,
+    const int* __restrict__ vlo, const int* __restrict__ vwid,
+    const unsigned* __restrict__ tilebits, const int* __restrict__ icol,
     const float* __restrict__ ival, const long long* __restrict__ vout, int* __restrict__ ccol,
     float* __restrict__ cval, long long* __restrict__ cnt_out) {
     constexpr int WMAX = 1 << kTileMaxLog2;
@@ -1791,6 +1802,7 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
     static_assert(NW <= kPT, "one bitmap word per thread");
     __shared__ double acc[WMAX];
     __shared__ unsigned bits[NW];
+    __shared__ unsigned short pre[NW];  // rank before each word
     __shared__ int sh[kPT / 64];
     for (int x = threadIdx.x; x < WMAX; x += kPT) acc[x] = 0.0;
     for (int w = threadIdx.x; w < NW; w += kPT) bits[w] = 0u;
@@ -1801,38 +1813,48 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
     int v = vlist[j];
     long long s0 = vbase[v], s1 = vbase[v + 1], o0 = vout[v], lim = vout[v + 1];
     int lo = vlo[v], nw = (vwid[v] + 31) >> 5;
-    int4 qc = make_int4(0, 0, 0, 0);
-    float4 qv = make_float4(0.f, 0.f, 0.f, 0.f);
+    // rows of the class are non-empty; a 16-byte group past the row's end
+    // is clamped to the group holding its last product (the intermediates
+    // are padded to whole groups), and its lanes are masked where used
+    int4 qc;
+    float4 qv;
     {
-        const long long b0 = (s0 & ~3LL) + (long long)threadIdx.x * 4;
-        if (b0 < s1) {
-            qc = *(const int4*)(icol + b0);
-            qv = *(const float4*)(ival + b0);
-        }
+        const long long b0 = min((s0 & ~3LL) + (long long)threadIdx.x * 4, (s1 - 1) & ~3LL);
+        qc = *(const int4*)(icol + b0);
+        qv = *(const float4*)(ival + b0);
     }
     for (;;) {
         const long long jn = j + G;
         const int vn = jn < nlist ? vlist[jn] : -1;
-        // four consecutive products per thread and load (16-byte loads)
+        unsigned wb = 0u;  // this row's bitmap word (exact path)
+        if (!BUILD && threadIdx.x < nw) wb = tilebits[(long long)v * kTileWords + threadIdx.x];
         for (long long sb = s0 & ~3LL;;) {
             const long long bb = sb + (long long)threadIdx.x * 4;
+            const long long nsb = sb + 4 * kPT;
+            int4 nc = qc;
+            float4 nv = qv;
+            if (nsb < s1) {  // uniform: the next group of the row, in flight
+                const long long nb = min(nsb + (long long)threadIdx.x * 4, (s1 - 1) & ~3LL);
+                nc = *(const int4*)(icol + nb);
+                nv = *(const float4*)(ival + nb);
+            }
             const int cc[4] = {qc.x, qc.y, qc.z, qc.w};
             const float xx[4] = {qv.x, qv.y, qv.z, qv.w};
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 if (bb + e >= s0 && bb + e < s1) {
                     const int c = cc[e] - lo;
-                    atomicOr(&bits[c >> 5], 1u << (c & 31));
+                    if constexpr (BUILD) atomicOr(&bits[c >> 5], 1u << (c & 31));
                     unsafeAtomicAdd(&acc[c], (double)xx[e]);
                 }
             }
-            sb += 4 * kPT;
+            qc = nc;
+            qv = nv;
+            sb = nsb;
             if (sb >= s1) break;
-            const long long nb = sb + (long long)threadIdx.x * 4;
-            if (nb < s1) {
-                qc = *(const int4*)(icol + nb);
-                qv = *(const float4*)(ival + nb);
-            }
+        }
+        if constexpr (!BUILD) {
+            if (threadIdx.x < NW) bits[threadIdx.x] = wb;  // zero past the window
         }
         __syncthreads();
         long long s0n = 0, s1n = 0, o0n = 0, limn = 0;
@@ -1844,31 +1866,34 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
             limn = vout[vn + 1];
             lon = vlo[vn];
             nwn = (vwid[vn] + 31) >> 5;
-            const long long nb = (s0n & ~3LL) + (long long)threadIdx.x * 4;
-            if (nb < s1n) {
-                qc = *(const int4*)(icol + nb);
-                qv = *(const float4*)(ival + nb);
-            }
+            const long long nb =
+                min((s0n & ~3LL) + (long long)threadIdx.x * 4, (s1n - 1) & ~3LL);
+            qc = *(const int4*)(icol + nb);
+            qv = *(const float4*)(ival + nb);
         }
-        unsigned b = threadIdx.x < nw ? bits[threadIdx.x] : 0u;
+        const unsigned b = threadIdx.x < NW ? bits[threadIdx.x] : 0u;
         int tot;
-        const int pre = block_exscan_n<int, kPT>(__popc(b), &tot, sh);
+        const int pw = block_exscan_n<int, kPT>(__popc(b), &tot, sh);
+        if (threadIdx.x < NW) pre[threadIdx.x] = (unsigned short)pw;
         if (cnt_out && threadIdx.x == 0) cnt_out[v] = tot;
-        long long o = o0 + pre;
-        while (b) {
-            const int bit = __ffs(b) - 1;
-            b &= b - 1;
-            const int cc = (threadIdx.x << 5) + bit;
-            const float val = (float)acc[cc];
-            acc[cc] = 0.0;
-            if (o < lim) {  // slot range fixed by the symbolic pass
-                ccol[o] = lo + cc;
-                cval[o] = val;
-            }
-            ++o;
-        }
-        if (threadIdx.x < nw) bits[threadIdx.x] = 0u;
         __syncthreads();
+        for (int col = threadIdx.x; col < nw * 32; col += kPT) {
+            const int w = col >> 5;
+            const unsigned bw = bits[w];
+            if ((bw >> (col & 31)) & 1u) {
+                const long long o = o0 + pre[w] + __popc(bw & ((1u << (col & 31)) - 1u));
+                if (o < lim) {  // slot range fixed by the symbolic pass
+                    ccol[o] = lo + col;
+                    cval[o] = (float)acc[col];
+                }
+                acc[col] = 0.0;
+            }
+        }
+        __syncthreads();
+        if constexpr (BUILD) {  // cleared once every thread has emitted (the next row ORs into it)
+            if (threadIdx.x < NW) bits[threadIdx.x] = 0u;
+            __syncthreads();
+        }
         if (vn < 0) break;
         j = jn;
         v = vn;
@@ -2028,16 +2053,20 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
             const int wpt = (((nw + kPT - 1) / kPT) + 1) & ~1;
             const int w0 = threadIdx.x * wpt;
             const int w1 = min(w0 + wpt, nw);
-            // trip count uniform across the workgroup (the OR below shuffles)
+            // trip count uniform across the workgroup (the OR below shuffles);
+            // positions past the row read its last product (rows of the
+            // class are non-empty) and are masked after the loads, so the
+            // loads do not wait for each other
             for (long long sb = s0; sb < s1; sb += 4 * kPT) {
                 int c[4];
     #pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    c[u] = icol[min(sb + threadIdx.x + (long long)u * kPT, s1 - 1)];
+    #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const long long q = sb + threadIdx.x + (long long)u * kPT;
-                    c[u] = q < s1 ? icol[q] - lo : -1;
+                    wave_bitmap_or(bits, q < s1 ? c[u] - lo : -1);
                 }
-    #pragma unroll
-                for (int u = 0; u < 4; ++u) wave_bitmap_or(bits, c[u]);
             }
             for (int q = threadIdx.x; q < cnt; q += kPT) acc[q] = 0.0;
             __syncthreads();
@@ -2056,40 +2085,233 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
                 float x[4];
     #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const long long q = s + (long long)u * kPT;
-                    c[u] = -1;
-                    x[u] = 0.f;
-                    if (q < s1) {
-                        c[u] = icol[q] - lo;
-                        x[u] = ival[q];
-                    }
+                    const long long q = min(s + (long long)u * kPT, s1 - 1);
+                    c[u] = icol[q];
+                    x[u] = ival[q];
                 }
     #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    if (c[u] >= 0) {
-                        const int w = c[u] >> 5;
+                    if (s + (long long)u * kPT < s1) {
+                        const int cu = c[u] - lo;
+                        const int w = cu >> 5;
                         const int r = pre[w >> 1] + ((w & 1) ? __popc(bits[w - 1]) : 0) +
-                                      __popc(bits[w] & ((1u << (c[u] & 31)) - 1u));
+                                      __popc(bits[w] & ((1u << (cu & 31)) - 1u));
                         if (r < cnt) unsafeAtomicAdd(&acc[r], (double)x[u]);
                     }
                 }
             }
             __syncthreads();
-            for (int w = w0; w < w1; ++w) {
-                unsigned b = bits[w];
-                int r = pre[w >> 1] + ((w & 1) ? __popc(bits[w - 1]) : 0);
-                while (b) {
-                    const int bit = __ffs(b) - 1;
-                    b &= b - 1;
-                    if (r < cnt) {
-                        ccol[o + r] = lo + (w << 5) + bit;
-                        cval[o + r] = (float)acc[r];
+            // emit: word pairs interleaved over the threads (neighbouring
+            // threads read neighbouring accumulators and write neighbouring
+            // output positions); a thread clears the pair it emitted
+            for (int pw = threadIdx.x; 2 * pw < nw; pw += kPT) {
+                int r = pre[pw];
+    #pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int w = 2 * pw + h;
+                    unsigned b = bits[w];  // words past nw are zero
+                    while (b) {
+                        const int bit = __ffs(b) - 1;
+                        b &= b - 1;
+                        if (r < cnt) {
+                            ccol[o + r] = lo + (w << 5) + bit;
+                            cval[o + r] = (float)acc[r];
+                        }
+                        ++r;
                     }
-                    ++r;
+                    bits[w] = 0u;
                 }
             }
-            for (int w = w0; w < w1; ++w) bits[w] = 0u;  // a thread emits and clears its own words
             __syncthreads();
+        }
+    }
+}
+
+// Rank rows whose window is at most 2^16 columns wide and whose count is at
+// most 1024, one wave per row: the same bitmap / prefix-popcount / FP64
+// accumulation as k_merge_rank, but a wave owns its bitmap (8 KB), so a CU
+// keeps several rows in flight instead of one workgroup-synchronized row.
+// A wave takes G <= 64 consecutive rows of the list (G = 64 when there are
+// rows enough to fill the chip): each lane loads one row's descriptor (one
+// memory round trip for G rows), and while a row is merged the next row's
+// products are already being loaded (16 per lane held in registers; a
+// longer row reads the rest in both passes).  No load sits under a
+// lane-divergent branch and no loaded value is used before the row that
+// needs it, so the loads issue back to back and are waited for once.
+// Ranks come from the pair prefixes, and the columns are kept beside the
+// accumulators so both outputs are written coalesced.  cnt_out (may be
+// null) receives the row's count; a row whose count exceeds its output
+// range writes only the range's first part.
+constexpr int kRankWavePer = 16;  // products per lane held in registers
+
+__device__ __forceinline__ long long readlane64(long long x, int r) {
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned long long)x, r);
+    const int hi = __builtin_amdgcn_readlane((int)((unsigned long long)x >> 32), r);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// The first 16 * 64 products of a non-empty row [s0, s1): lane l gets
+// products s0 + u * 64 + l; a position past the row reads the row's last
+// product instead (its lane is masked where the value is used).
+__device__ __forceinline__ void rank_wave_load(const int* __restrict__ icol,
+                                               const float* __restrict__ ival, long long s0,
+                                               long long s1, int* c, float* x) {
+    const int lane = lane_id();
+    const long long last = s1 - 1;
+#pragma unroll
+    for (int u = 0; u < kRankWavePer; ++u) {
+        long long q = s0 + u * 64 + lane;
+        q = q < s1 ? q : last;
+        c[u] = icol[q];
+        x[u] = ival[q];
+    }
+}
+
+// Rank of window column cc: set columns before it (pair prefix + the
+// words of its pair before it + the bits of its word below it).
+__device__ __forceinline__ int rank_of(const unsigned* bits, const unsigned short* pre, int cc) {
+    const int w = cc >> 5;
+    const uint2 bb = *(const uint2*)(bits + (w & ~1));
+    const unsigned low = (1u << (cc & 31)) - 1u;
+    return (int)pre[w >> 1] + ((w & 1) ? __popc(bb.x) + __popc(bb.y & low) : __popc(bb.x & low));
+}
+
+template <int NW, int CAP>
+__global__ __launch_bounds__(64) void k_merge_rank_wave(
+    const int* __restrict__ vlist, long long nlist, int G, const long long* __restrict__ vbase,
+    const int* __restrict__ vlo, const int* __restrict__ vwid, const int* __restrict__ icol,
+    const float* __restrict__ ival, const long long* __restrict__ vout, int* __restrict__ ccol,
+    float* __restrict__ cval, long long* __restrict__ cnt_out) {
+    constexpr int PER = kRankWavePer;
+    static_assert(NW % 2 == 0, "words are handled in pairs");
+    __shared__ __align__(8) unsigned bits[NW];
+    __shared__ unsigned short pre[NW / 2];  // rank before each word pair
+    __shared__ double acc[CAP];
+    __shared__ int colr[CAP];
+    const int lane = lane_id();
+    for (int w = lane; w < NW; w += 64) bits[w] = 0u;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const long long ngrp = (nlist + G - 1) / G;
+    for (long long g = blockIdx.x; g < ngrp; g += gridDim.x) {
+        // descriptors of the group's rows, one per lane
+        const long long jl = g * G + lane;
+        int mv = 0, mcnt = 0, mlo = 0, mnw = 0;
+        long long ms0 = 0, ms1 = 0, mo = 0;
+        if (lane < G && jl < nlist) {
+            mv = vlist[jl];
+            ms0 = vbase[mv];
+            ms1 = vbase[mv + 1];
+            mo = vout[mv];
+            mcnt = (int)min((long long)CAP, vout[mv + 1] - mo);
+            mlo = vlo[mv];
+            mnw = (vwid[mv] + 31) >> 5;  // <= NW (class condition)
+        }
+        // the descriptors are complete before the row loop (else every row
+        // would wait there for the previous row's stores as well)
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        const int nr = (int)min((long long)G, nlist - g * G);
+        int c[PER];
+        float x[PER];
+        rank_wave_load(icol, ival, readlane64(ms0, 0), readlane64(ms1, 0), c, x);
+        for (int r = 0; r < nr; ++r) {
+            const int v = __builtin_amdgcn_readlane(mv, r);
+            const long long s0 = readlane64(ms0, r), s1 = readlane64(ms1, r);
+            const long long o0 = readlane64(mo, r);
+            const int cnt = __builtin_amdgcn_readlane(mcnt, r);
+            const int lo = __builtin_amdgcn_readlane(mlo, r);
+            const int nw = __builtin_amdgcn_readlane(mnw, r);
+            const long long base = s0 + lane;
+            // the next row's products (the last row loads its own again)
+            int cn[PER];
+            float xn[PER];
+            const int rn = r + 1 < nr ? r + 1 : r;
+            rank_wave_load(icol, ival, readlane64(ms0, rn), readlane64(ms1, rn), cn, xn);
+            // pass 1: the row's column bitmap
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                if (s0 + u * 64 < s1) {  // uniform
+                    const bool ok = base + u * 64 < s1;
+                    const int cc = c[u] - lo;
+                    atomicOr(&bits[ok ? cc >> 5 : 0], ok ? 1u << (cc & 31) : 0u);
+                }
+            }
+            for (long long q = s0 + PER * 64 + lane; q < s1; q += 64) {
+                const int cc = icol[q] - lo;
+                atomicOr(&bits[cc >> 5], 1u << (cc & 31));
+            }
+            for (int q = lane; q < cnt; q += 64) acc[q] = 0.0;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            // prefix popcounts over word pairs, an even number of words per
+            // lane (words past nw are zero)
+            const int wpt = (((nw + 63) >> 6) + 1) & ~1;
+            const int w0 = lane * wpt, w1 = min(w0 + wpt, nw);
+            int mine = 0;
+            for (int w = w0; w < w1; w += 2) {
+                const uint2 bb = *(const uint2*)(bits + w);
+                mine += __popc(bb.x) + __popc(bb.y);
+            }
+            int inc = mine;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += y;
+            }
+            const int total = __shfl(inc, 63, 64);
+            if (cnt_out && lane == 0) cnt_out[v] = total;
+            int run = inc - mine;
+            for (int w = w0; w < w1; w += 2) {
+                const uint2 bb = *(const uint2*)(bits + w);
+                pre[w >> 1] = (unsigned short)run;
+                run += __popc(bb.x) + __popc(bb.y);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            // pass 2: ranks first (all reads in flight together), then the
+            // FP64 accumulation and the column of every rank
+            int rk[PER];
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                rk[u] = CAP;
+                if (s0 + u * 64 < s1) {
+                    const bool ok = base + u * 64 < s1;
+                    const int r0 = rank_of(bits, pre, ok ? c[u] - lo : 0);
+                    rk[u] = ok ? r0 : CAP;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                if (s0 + u * 64 < s1 && rk[u] < cnt) {
+                    unsafeAtomicAdd(&acc[rk[u]], (double)x[u]);
+                    colr[rk[u]] = c[u];  // every product of a column stores the same value
+                }
+            }
+            for (long long q = s0 + PER * 64 + lane; q < s1; q += 64) {
+                const int cc = icol[q] - lo;
+                const int r0 = rank_of(bits, pre, cc);
+                if (r0 < cnt) {
+                    unsafeAtomicAdd(&acc[r0], (double)ival[q]);
+                    colr[r0] = cc + lo;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            // the next row's products (this waits for their loads, not for
+            // the stores below)
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                c[u] = cn[u];
+                x[u] = xn[u];
+            }
+            const int ne = min(cnt, total);
+            for (int q = lane; q < ne; q += 64) {
+                ccol[o0 + q] = colr[q];
+                cval[o0 + q] = (float)acc[q];
+            }
+            for (int w = w0; w < w1; ++w) bits[w] = 0u;  // a lane clears the words it counted
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
         }
     }
 }
@@ -2138,8 +2360,22 @@ static int launch_merge_class(spg_context* c, int b, const int* list, long long 
                                list, cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval,
                                o.cnt);
     } else if (b == kClassTile) {
-        hipLaunchKernelGGL(k_merge_tile, dim3(persistent_grid(c, cnt, 2)), dim3(kPT), 0,
-                           c->stream, list, cnt, vbase, vlo, (const int*)c->vwid.p, icol, ival,
+        const dim3 g(persistent_grid(c, cnt, 2));
+        if (build)
+            hipLaunchKernelGGL(k_merge_tile<true>, g, dim3(kPT), 0, c->stream, list, cnt, vbase,
+                               vlo, (const int*)c->vwid.p, tb, icol, ival, o.obase, o.ocol,
+                               o.oval, o.cnt);
+        else
+            hipLaunchKernelGGL(k_merge_tile<false>, g, dim3(kPT), 0, c->stream, list, cnt, vbase,
+                               vlo, (const int*)c->vwid.p, tb, icol, ival, o.obase, o.ocol,
+                               o.oval, o.cnt);
+    } else if (b == kClassRankW) {
+        // rows per wave: 64 when the list fills 7 waves per CU that many times
+        const long long waves = (long long)c->num_cu * 7;
+        const int G = (int)std::min<long long>(64, std::max<long long>(1, cnt / waves));
+        hipLaunchKernelGGL((k_merge_rank_wave<kRankWaveWords, kRankWaveCap>),
+                           dim3(persistent_grid(c, (cnt + G - 1) / G, 7)), dim3(64), 0,
+                           c->stream, list, cnt, G, vbase, vlo, (const int*)c->vwid.p, icol, ival,
                            o.obase, o.ocol, o.oval, o.cnt);
     } else if (b == kClassRank) {
         hipLaunchKernelGGL(k_merge_rank,
@@ -2253,7 +2489,8 @@ __global__ void k_learned_rowest(int m, const int* __restrict__ vstart,
 // the tile and hash classes; the raw prediction decides rank vs hash, as the
 // exact count does in the plain merge); the capacity is
 // the most that class's kernel can hold (256 or 1024 columns for the wave
-// tile kernels, the hash table's half for hash rows) or the upper bound
+// tile kernels, 1024 for the wave rank kernel, the hash table's half for
+// hash rows) or the upper bound
 // min(products, width) for the dense tile and rank kernels, so a row
 // overflows only when its count exceeds its class (with tight, the inflated
 // prediction itself, which saves memory).  Capacity costs memory,
@@ -2298,6 +2535,9 @@ __global__ void k_learned_class(long long V, const int* __restrict__ vrow_row,
                     k = kClassTile;
                     cap = bound;
                 }
+            } else if (rank_ok && width <= kRankWaveWords * 32 && wl <= kRankWaveCap) {
+                k = kClassRankW;
+                cap = min(bound, (long long)kRankWaveCap);
             } else if (rank_ok && width <= kRankMaxWords * 32) {
                 k = kClassRank;
                 cap = bound;  // <= 4096 = the rank kernel's accumulator

@@ -20,7 +20,7 @@ constexpr int kTileMaxLog2 = 13;        // dense tile: 8192 FP64 accumulators
 constexpr int kHashMinLog2 = 6;         // smallest hash: 64 slots
 // Merge/symbolic classes of a virtual row.
 constexpr int kNumHashClasses = kHashMaxLog2 - kHashMinLog2 + 1;  // 64..8192
-constexpr int kNumVClasses = kNumHashClasses + 4;  // + tile (large), rank, tile small/medium
+constexpr int kNumVClasses = kNumHashClasses + 5;  // + tile (large), rank, tile small/medium, rank (wave)
 
 // Grow-only device buffer.
 struct DevBuf {
