@@ -802,6 +802,24 @@ __device__ void expand_work(const ExpandArgs& g, ExpandLds& L, int k, long long 
     if (T > 1)
         for (int t = threadIdx.x; t <= T; t += kBlock) L.sbs[t] = g.bsplit[(long long)k * (T + 1) + t];
     const int jlo = b0 + j0, jhi = b0 + j0 + nj;
+    // Every A entry of the item is scaled against the same piece of B row k,
+    // and a lane serves the same positions of it for every entry (lane l of
+    // a G-lane group: l, l+G, l+2G, l+3G), so pieces of at most 4G columns
+    // are loaded into registers once per item instead of once per entry.
+    const int lj = threadIdx.x & (G - 1);
+    const bool breg = nj <= 4 * G;
+    int bc[4] = {0, 0, 0, 0};
+    float bx[4] = {0.f, 0.f, 0.f, 0.f};
+    if (breg) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (u * G < nj) {  // uniform; nj >= 1, positions past it clamped
+                const int q = min(jlo + lj + u * G, jhi - 1);
+                bc[u] = g.b_idx[q];
+                bx[u] = g.b_val[q];
+            }
+        }
+    }
     for (long long eb = 0; eb < ne; eb += R) {
         const int cnt = (int)min((long long)R, ne - eb);
         if (eb > 0) __syncthreads();
@@ -833,6 +851,11 @@ __device__ void expand_work(const ExpandArgs& g, ExpandLds& L, int k, long long 
             }
             __syncthreads();
         }
+        // every load before this point is complete here (the slot
+        // reservation already waited for its atomics, which are issued after
+        // them), said explicitly so that the stores below do not each wait
+        // for the stores before them
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
         for (int sb = 0; sb < cnt; sb += NE) {  // writes, NE entries at a time
             const int q = sb + (threadIdx.x >> lg);
             if (q >= cnt) continue;
@@ -843,7 +866,21 @@ __device__ void expand_work(const ExpandArgs& g, ExpandLds& L, int k, long long 
             const float* __restrict__ bval = g.b_val;
             int* __restrict__ icol = g.icol;
             float* __restrict__ ival = g.ival;
-            for (int jj0 = threadIdx.x & (G - 1); jj0 < nj; jj0 += 4 * G) {
+            if (breg) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int jj = lj + u * G;
+                    if (jj < nj) {
+                        long long slot;
+                        if (!fl) slot = base + jj;
+                        else slot = L.tbase[q * T + (bc[u] >> g.tile_log2) - tfirst] + jlo + jj;
+                        icol[slot] = bc[u];
+                        ival[slot] = __fmul_rn(a, bx[u]);
+                    }
+                }
+                continue;
+            }
+            for (int jj0 = lj; jj0 < nj; jj0 += 4 * G) {
                 int col[4];
                 float bv[4];
 #pragma unroll
@@ -1937,6 +1974,16 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
         const long long o0 = vout[v];
         int cnt = (int)min((long long)CAP, vout[v + 1] - o0);
         const int lo = vlo[v];
+        // the first products of the accumulation pass, in flight from here:
+        // 16-byte groups past the row's end are clamped to the group holding
+        // its last product (rows of the class are non-empty; the
+        // intermediates are padded to whole groups) and masked where used,
+        // so no load sits under a lane-divergent branch; later the next
+        // group is in flight while the current one is applied
+        const long long glast = (s1 - 1) & ~3LL;
+        long long ga = min((s0 & ~3LL) + (long long)lane * 4, glast);
+        int4 qc = *(const int4*)(icol + ga);
+        float4 qv = *(const float4*)(ival + ga);
         uint4 wq;
         if constexpr (BUILD) {
             *(uint4*)(bits + lane * 4) = make_uint4(0u, 0u, 0u, 0u);
@@ -1944,7 +1991,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
             __builtin_amdgcn_wave_barrier();
             for (long long sb = s0 & ~3LL; sb < s1; sb += 4 * 64) {
                 const long long base = sb + (long long)lane * 4;
-                const int4 q = base < s1 ? *(const int4*)(icol + base) : make_int4(0, 0, 0, 0);
+                const int4 q = *(const int4*)(icol + min(base, (s1 - 1) & ~3LL));
                 const int cc[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
@@ -1983,13 +2030,15 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
         for (int q = lane; q < cnt; q += 64) acc[q] = 0.0;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        for (long long sb = s0 & ~3LL; sb < s1; sb += 4 * 64) {
+        for (long long sb = s0 & ~3LL;;) {
             const long long base = sb + (long long)lane * 4;
-            int4 qc = make_int4(0, 0, 0, 0);
-            float4 qv = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (base < s1) {
-                qc = *(const int4*)(icol + base);
-                qv = *(const float4*)(ival + base);
+            const long long nsb = sb + 4 * 64;
+            int4 nc = qc;
+            float4 nv = qv;
+            if (nsb < s1) {  // uniform
+                ga = min(nsb + (long long)lane * 4, glast);
+                nc = *(const int4*)(icol + ga);
+                nv = *(const float4*)(ival + ga);
             }
             const int cc[4] = {qc.x, qc.y, qc.z, qc.w};
             const float xx[4] = {qv.x, qv.y, qv.z, qv.w};
@@ -2005,6 +2054,10 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
                     }
                 }
             }
+            qc = nc;
+            qv = nv;
+            sb = nsb;
+            if (sb >= s1) break;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
