@@ -203,7 +203,7 @@ int spgDestroy(spg_context* c) {
                       &c->bsplit,
                       &c->vub,
                       &c->vbase, &c->cursor, &c->kbin, &c->kperm, &c->kbin_off, &c->kitems,
-                      &c->kitem_off, &c->icol, &c->ival, &c->vclass, &c->vlist,
+                      &c->kitem_off, &c->inter, &c->vclass, &c->vlist,
                       &c->vclass_off, &c->vcnt, &c->vout, &c->crow, &c->ccol, &c->cval,
                       &c->pred, &c->vcap, &c->pbase, &c->fbase, &c->pcol, &c->pval,
                       &c->fcol, &c->fval, &c->rwinb, &c->kwinb, &c->scratch};
@@ -448,8 +448,7 @@ int spgInitGEMM(spg_context* c, cm* C, int64_t* nprod) {
     int64_t np = 0;
     SPG_TRY(read_scalar<int64_t>(c, (const int64_t*)c->vbase.p + c->V, &np));
     c->nprod = np;
-    SPG_TRY(ensure(c, c->icol, sizeof(int32_t) * (size_t)(np + 16)));  // 16-byte loads read past the end
-    SPG_TRY(ensure(c, c->ival, sizeof(float) * (size_t)(np + 16)));
+    SPG_TRY(ensure(c, c->inter, sizeof(int32_t) * inter_words(np)));
     tmark(c, SPG_PHASE_INIT, 1);
     if (C) {
         cmInit(C);
@@ -675,8 +674,16 @@ int spgGetIntermediateByRow(spg_context* c, int64_t* row_ptr, int32_t* cols, flo
         SPG_TRY(launch_regroup_intermediate(c, d));
         SPG_TRY(copy_d2h(c, row_ptr, d, sizeof(int64_t) * ((size_t)c->m + 1)));
     }
-    if (cols) SPG_TRY(copy_d2h(c, cols, c->icol.p, sizeof(int32_t) * (size_t)c->nprod));
-    if (vals) SPG_TRY(copy_d2h(c, vals, c->ival.p, sizeof(float) * (size_t)c->nprod));
+    if (cols || vals) {  // the grouped layout, split into the two arrays
+        const long long np = c->nprod;
+        std::vector<int32_t> w((size_t)((np + kIGroup - 1) / kIGroup) * (size_t)(2 * kIGroup));
+        if (!w.empty()) SPG_TRY(copy_d2h(c, w.data(), c->inter.p, sizeof(int32_t) * w.size()));
+        for (long long s = 0; s < np; ++s) {
+            const long long q = islot(s);
+            if (cols) cols[s] = w[(size_t)q];
+            if (vals) memcpy(&vals[s], &w[(size_t)(q + kIGroup)], sizeof(float));
+        }
+    }
     return SPG_OK;
 }
 

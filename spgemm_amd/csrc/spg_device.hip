@@ -8,8 +8,14 @@
 //   into T column tiles of width W = 2^tile_log2; every other row is one
 //   virtual row.  vstart[m+1] maps rows to their first virtual row, vrow_row
 //   maps back, vub/vbase give each virtual row's slot range.
-//   intermediates  icol[nprod] int32, ival[nprod] f32 (SoA), grouped by
-//   virtual row (so a C row's products are contiguous too)
+//   intermediates  one buffer of 128-byte groups of 16 products: the 16
+//   columns (int32), then the 16 values (f32); product s sits at word
+//   islot(s) of the column pointer and of the value pointer (16 words
+//   later).  A run of consecutive products fills whole groups, so the
+//   expansion's scattered runs complete twice as many cache lines as with
+//   two separate arrays, and the column-only symbolic pass reads only the
+//   column halves.  Products are grouped by virtual row (so a C row's
+//   products are contiguous too)
 //   C  CSR  crow[m+1] int64, ccol[nnzC] int32, cval[nnzC] f32
 //
 // Value contract: a product is the FP32 product a*b (rounded once, as
@@ -874,8 +880,9 @@ __device__ void expand_work(const ExpandArgs& g, ExpandLds& L, int k, long long 
                         long long slot;
                         if (!fl) slot = base + jj;
                         else slot = L.tbase[q * T + (bc[u] >> g.tile_log2) - tfirst] + jlo + jj;
-                        icol[slot] = bc[u];
-                        ival[slot] = __fmul_rn(a, bx[u]);
+                        const long long w = islot(slot);
+                        icol[w] = bc[u];
+                        ival[w] = __fmul_rn(a, bx[u]);
                     }
                 }
                 continue;
@@ -896,8 +903,9 @@ __device__ void expand_work(const ExpandArgs& g, ExpandLds& L, int k, long long 
                         long long slot;
                         if (!fl) slot = base + jj;
                         else slot = L.tbase[q * T + (col[u] >> g.tile_log2) - tfirst] + jlo + jj;
-                        icol[slot] = col[u];
-                        ival[slot] = __fmul_rn(a, bv[u]);
+                        const long long w = islot(slot);
+                        icol[w] = col[u];
+                        ival[w] = __fmul_rn(a, bv[u]);
                     }
                 }
             }
@@ -907,6 +915,16 @@ __device__ void expand_work(const ExpandArgs& g, ExpandLds& L, int k, long long 
 
 __device__ __forceinline__ int ceil_log2_6(int nj) {
     return min(6, 32 - __clz(max(nj, 1) - 1));
+}
+
+// Lanes per A entry (log2): the next power of two >= the B piece, at most a
+// wave; pieces longer than a wave get more lanes (up to the workgroup) when
+// the item has too few entries to give every wave one, so that no wave of
+// the workgroup idles while one writes a long piece.
+__device__ __forceinline__ int lanes_log2(int nj, long long ne) {
+    int lg = ceil_log2_6(nj);
+    while (lg < 8 && (long long)(kBlock >> lg) > ne && (1 << lg) < nj) ++lg;
+    return lg;
 }
 
 // Binned expansion: persistent workgroups take chunks of kItemChunk
@@ -958,7 +976,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs g,
                 j0 = (int)(local % jch) * g.P;
                 nj = min(g.P, blen - j0);
             }
-            expand_work(g, L, k, e_begin, ne, j0, nj, ceil_log2_6(nj));
+            expand_work(g, L, k, e_begin, ne, j0, nj, lanes_log2(nj, ne));
         }
     }
 }
@@ -971,7 +989,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_simple(ExpandArgs g) {
     const long long ne = g.a_ptr[k + 1] - a0;
     const int nj = g.b_ptr[k + 1] - b0;
     if (ne == 0 || nj == 0) return;
-    expand_work(g, L, k, a0, ne, 0, nj, 6);
+    expand_work(g, L, k, a0, ne, 0, nj, lanes_log2(max(nj, 64), ne));
 }
 
 static ExpandArgs expand_args(spg_context* c) {
@@ -993,8 +1011,8 @@ static ExpandArgs expand_args(spg_context* c) {
     g.T = c->have_long ? c->T : 1;
     g.tile_log2 = c->cfg.tile_log2;
     g.swizzle = c->cfg.xcd_swizzle;
-    g.icol = (int*)c->icol.p;
-    g.ival = (float*)c->ival.p;
+    g.icol = inter_cols(c);
+    g.ival = inter_vals(c);
     return g;
 }
 
@@ -1150,8 +1168,9 @@ __global__ __launch_bounds__(kPT) void k_expand_rows_wave(RowsArgs g, const int*
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     if (p[u] < total) {
-                        icol[dst + p[u]] = col[u];
-                        ival[dst + p[u]] = __fmul_rn(sa[u], bv[u]);
+                        const long long w = islot(dst + p[u]);
+                        icol[w] = col[u];
+                        ival[w] = __fmul_rn(sa[u], bv[u]);
                     }
                 }
             }
@@ -1216,8 +1235,9 @@ __global__ __launch_bounds__(kBlock) void k_expand_rows_wg(RowsArgs g, const int
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     if (p[u] < total) {
-                        g.icol[dst + p[u]] = col[u];
-                        g.ival[dst + p[u]] = __fmul_rn(sa[lo[u]], bv[u]);
+                        const long long w = islot(dst + p[u]);
+                        g.icol[w] = col[u];
+                        g.ival[w] = __fmul_rn(sa[lo[u]], bv[u]);
                     }
                 }
             }
@@ -1258,8 +1278,8 @@ int launch_expand_rows(spg_context* c) {
     g.vstart = (const int*)c->vstart.p;
     g.vbase = (const long long*)c->vbase.p;
     g.T = c->T;
-    g.icol = (int*)c->icol.p;
-    g.ival = (float*)c->ival.p;
+    g.icol = inter_cols(c);
+    g.ival = inter_vals(c);
     const int* vl = (const int*)c->vlist.p;
     tmark(c, SPG_PHASE_EXPAND, 0);
     if (n1 > 0) {  // largest rows first
@@ -1312,6 +1332,7 @@ __device__ __forceinline__ int hash_insert(int* keys, int key, bool& fresh) {
 constexpr int kClassTile = kNumHashClasses;       // 8: large tile rows (workgroup)
 constexpr int kClassRank = kNumHashClasses + 1;   // 9
 constexpr int kClassTileS = kNumHashClasses + 2;  // 10: tile rows, count <= 256 (wave)
+constexpr int kTileSMax = 256;                    // largest count of the small tile class
 constexpr int kClassTileM = kNumHashClasses + 3;  // 11: tile rows, count <= 1024 (wave)
 constexpr int kClassRankW = kNumHashClasses + 4;  // 12: rank rows of a narrow window (wave)
 constexpr int kTileWords = (1 << kTileMaxLog2) / 32;  // bitmap words of one tile
@@ -1360,7 +1381,7 @@ __global__ __launch_bounds__(kBlock) void k_symbolic_hash(const int* __restrict_
     int mine = 0;
     for (long long s = vbase[v] + threadIdx.x; s < s1; s += kBlock) {
         bool fresh;
-        hash_insert<LOG2H>(keys, icol[s], fresh);
+        hash_insert<LOG2H>(keys, icol[islot(s)], fresh);
         mine += fresh;
     }
     if (mine) atomicAdd(&cnt, mine);
@@ -1395,8 +1416,10 @@ __device__ __forceinline__ T block_exscan_n(T x, T* total, T* sh) {
 // Symbolic count of tile-width virtual rows: one wave per row with a
 // wave-private bitmap of the tile's columns (no workgroup barriers, so the
 // waves of a CU overlap their load latencies).  Products are read 16 bytes
-// per lane, the next row's descriptor is loaded while the current one is
-// counted, and the finished bitmap is stored for the merge pass.
+// per lane (column halves of the product groups), the next row's
+// descriptor is loaded while the current one is counted, and the finished
+// bitmap is stored for the merge pass when the row is larger than the small
+// tile class.
 constexpr int kSymWaves = kPT / 64;
 
 __global__ __launch_bounds__(kPT) void k_symbolic_tile(
@@ -1421,8 +1444,8 @@ __global__ __launch_bounds__(kPT) void k_symbolic_tile(
     int4 q[2];
 #pragma unroll
     for (int u = 0; u < 2; ++u)
-        q[u] = *(const int4*)(icol + min((s0 & ~3LL) + (long long)(lane + u * 64) * 4,
-                                         (s1 - 1) & ~3LL));
+        q[u] = *(const int4*)(icol + islot(min((s0 & ~3LL) + (long long)(lane + u * 64) * 4,
+                                               (s1 - 1) & ~3LL)));
     for (;;) {
         const long long jn = j + stride;
         const int vn = jn < nlist ? vlist[jn] : -1;
@@ -1432,8 +1455,8 @@ __global__ __launch_bounds__(kPT) void k_symbolic_tile(
             if (nsb < s1) {  // uniform
 #pragma unroll
                 for (int u = 0; u < 2; ++u)
-                    qn[u] = *(const int4*)(icol + min(nsb + (long long)(lane + u * 64) * 4,
-                                                      (s1 - 1) & ~3LL));
+                    qn[u] = *(const int4*)(icol + islot(min(nsb + (long long)(lane + u * 64) * 4,
+                                                            (s1 - 1) & ~3LL)));
             }
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
@@ -1460,17 +1483,19 @@ __global__ __launch_bounds__(kPT) void k_symbolic_tile(
             lon = vlo[vn];
 #pragma unroll
             for (int u = 0; u < 2; ++u)
-                q[u] = *(const int4*)(icol + min((s0n & ~3LL) + (long long)(lane + u * 64) * 4,
-                                                 (s1n - 1) & ~3LL));
+                q[u] = *(const int4*)(icol + islot(min((s0n & ~3LL) + (long long)(lane + u * 64) * 4,
+                                                       (s1n - 1) & ~3LL)));
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         static_assert(kTileWords == 4 * 64, "four bitmap words per lane");
         uint4 wq = *(const uint4*)(bits + lane * 4);
         int mine = __popc(wq.x) + __popc(wq.y) + __popc(wq.z) + __popc(wq.w);
-        *(uint4*)(tilebits + (long long)v * kTileWords + lane * 4) = wq;
         *(uint4*)(bits + lane * 4) = make_uint4(0u, 0u, 0u, 0u);
         mine = wave_sum(mine);
+        // rows of the small tile class rebuild their bitmap in the merge
+        // from their few products (cheaper than 1 KB stored and loaded)
+        if (mine > kTileSMax) *(uint4*)(tilebits + (long long)v * kTileWords + lane * 4) = wq;
         if (lane == 0) vcnt[v] = mine;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -1513,7 +1538,7 @@ __global__ __launch_bounds__(64 * WPB) void k_symbolic_hash_wave(
         int mine = 0;
         for (long long s = vbase[v] + lane; s < s1; s += 64) {
             bool fresh;
-            hash_insert<LOG2H>(keys, icol[s], fresh);
+            hash_insert<LOG2H>(keys, icol[islot(s)], fresh);
             mine += fresh;
         }
         mine = wave_sum(mine);
@@ -1558,9 +1583,9 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_hash_wave(
         bool full = false;
         for (long long s = vbase[v] + lane; s < s1; s += 64) {
             bool fresh;
-            const int h = hash_insert<LOG2H>(keys, icol[s], fresh);
+            const int h = hash_insert<LOG2H>(keys, icol[islot(s)], fresh);
             if (h >= 0)
-                unsafeAtomicAdd(&vals[h], (double)ival[s]);
+                unsafeAtomicAdd(&vals[h], (double)ival[islot(s)]);
             else
                 full = true;
         }
@@ -1617,7 +1642,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_hash_wave(
 template <int LOG2H>
 static void launch_sym_hash(spg_context* c, const int* list, long long cnt) {
     hipLaunchKernelGGL(k_symbolic_hash<LOG2H>, dim3((unsigned)cnt), dim3(kBlock), 0, c->stream,
-                       list, (const long long*)c->vbase.p, (const int*)c->icol.p,
+                       list, (const long long*)c->vbase.p, (const int*)inter_cols(c),
                        (long long*)c->vcnt.p);
 }
 
@@ -1650,13 +1675,13 @@ int launch_symbolic(spg_context* c) {
                                dim3(persistent_grid(c, (cnt + kSymWaves - 1) / kSymWaves, 4)),
                                dim3(kPT), 0, c->stream, list, cnt, (const long long*)c->vbase.p,
                                (const int*)c->vlo.p, (const int*)c->vwid.p,
-                               (const int*)c->icol.p, (long long*)c->vcnt.p,
+                               (const int*)inter_cols(c), (long long*)c->vcnt.p,
                                (unsigned*)c->tilebits.p);
         } else {
             const dim3 gw(persistent_grid(c, (cnt + kHashWaveWPB - 1) / kHashWaveWPB, 4));
             const dim3 bw(64 * kHashWaveWPB);
             const long long* vb = (const long long*)c->vbase.p;
-            const int* ic = (const int*)c->icol.p;
+            const int* ic = (const int*)inter_cols(c);
             long long* vc = (long long*)c->vcnt.p;
             switch (b + kHashMinLog2) {
                 case 6:
@@ -1714,7 +1739,7 @@ __global__ void k_vclass_merge(long long V, const long long* __restrict__ vcnt,
     }
     const int width = vwid[v];
     if (width <= W) {
-        cls[v] = cnt <= 256 ? kClassTileS : (cnt <= 1024 ? kClassTileM : kClassTile);
+        cls[v] = cnt <= kTileSMax ? kClassTileS : (cnt <= 1024 ? kClassTileM : kClassTile);
         return;
     }
     if (cnt > rank_min && width <= kRankWaveWords * 32 && cnt <= kRankWaveCap) {
@@ -1760,9 +1785,9 @@ __global__ __launch_bounds__(kBlock) void k_merge_hash(
     const long long s1 = vbase[v + 1];
     for (long long s = vbase[v] + threadIdx.x; s < s1; s += kBlock) {
         bool fresh;
-        const int h = hash_insert<LOG2H>(keys, icol[s], fresh);
+        const int h = hash_insert<LOG2H>(keys, icol[islot(s)], fresh);
         if (h >= 0)
-            unsafeAtomicAdd(&vals[h], (double)ival[s]);
+            unsafeAtomicAdd(&vals[h], (double)ival[islot(s)]);
         else
             full = 1;
     }
@@ -1819,9 +1844,9 @@ __global__ __launch_bounds__(kBlock) void k_merge_hash(
 // Tile-width virtual rows: dense FP64 accumulator over the tile's columns
 // plus an occupancy bitmap (stored by the symbolic pass, or built here with
 // BUILD when none ran).  Persistent workgroups walk the list with a fixed
-// stride; each thread loads four consecutive products at a time (16-byte
-// loads) and the next four are in flight while the current ones are
-// accumulated; the next row's descriptor and first products are loaded
+// stride; each thread loads two groups of four consecutive products per
+// pass (16-byte loads) and the next pass's groups are in flight while the
+// current ones are accumulated; the next row's descriptor and first products are loaded
 // before the current row is emitted.  The emit walks the columns in order,
 // one per thread (consecutive threads read consecutive accumulators, so no
 // LDS bank is hit twice, and write consecutive output positions); its rank
@@ -1853,12 +1878,17 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
     // rows of the class are non-empty; a 16-byte group past the row's end
     // is clamped to the group holding its last product (the intermediates
     // are padded to whole groups), and its lanes are masked where used
-    int4 qc;
-    float4 qv;
-    {
-        const long long b0 = min((s0 & ~3LL) + (long long)threadIdx.x * 4, (s1 - 1) & ~3LL);
-        qc = *(const int4*)(icol + b0);
-        qv = *(const float4*)(ival + b0);
+    // D groups of four products per thread and pass (2 * 16 KB in flight per
+    // workgroup while the previous pass is applied)
+    constexpr int D = 2;
+    int4 qc[D];
+    float4 qv[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const long long b0 =
+            min((s0 & ~3LL) + (long long)(d * kPT + threadIdx.x) * 4, (s1 - 1) & ~3LL);
+        qc[d] = *(const int4*)(icol + islot(b0));
+        qv[d] = *(const float4*)(ival + islot(b0));
     }
     for (;;) {
         const long long jn = j + G;
@@ -1866,27 +1896,42 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
         unsigned wb = 0u;  // this row's bitmap word (exact path)
         if (!BUILD && threadIdx.x < nw) wb = tilebits[(long long)v * kTileWords + threadIdx.x];
         for (long long sb = s0 & ~3LL;;) {
-            const long long bb = sb + (long long)threadIdx.x * 4;
-            const long long nsb = sb + 4 * kPT;
-            int4 nc = qc;
-            float4 nv = qv;
-            if (nsb < s1) {  // uniform: the next group of the row, in flight
-                const long long nb = min(nsb + (long long)threadIdx.x * 4, (s1 - 1) & ~3LL);
-                nc = *(const int4*)(icol + nb);
-                nv = *(const float4*)(ival + nb);
-            }
-            const int cc[4] = {qc.x, qc.y, qc.z, qc.w};
-            const float xx[4] = {qv.x, qv.y, qv.z, qv.w};
+            const long long nsb = sb + D * 4 * kPT;
+            int4 nc[D];
+            float4 nv[D];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                if (bb + e >= s0 && bb + e < s1) {
-                    const int c = cc[e] - lo;
-                    if constexpr (BUILD) atomicOr(&bits[c >> 5], 1u << (c & 31));
-                    unsafeAtomicAdd(&acc[c], (double)xx[e]);
+            for (int d = 0; d < D; ++d) {
+                nc[d] = qc[d];
+                nv[d] = qv[d];
+            }
+            if (nsb < s1) {  // uniform: the next groups of the row, in flight
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const long long nb =
+                        min(nsb + (long long)(d * kPT + threadIdx.x) * 4, (s1 - 1) & ~3LL);
+                    nc[d] = *(const int4*)(icol + islot(nb));
+                    nv[d] = *(const float4*)(ival + islot(nb));
                 }
             }
-            qc = nc;
-            qv = nv;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const long long bb = sb + (long long)(d * kPT + threadIdx.x) * 4;
+                const int cc[4] = {qc[d].x, qc[d].y, qc[d].z, qc[d].w};
+                const float xx[4] = {qv[d].x, qv[d].y, qv[d].z, qv[d].w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    if (bb + e >= s0 && bb + e < s1) {
+                        const int c = cc[e] - lo;
+                        if constexpr (BUILD) atomicOr(&bits[c >> 5], 1u << (c & 31));
+                        unsafeAtomicAdd(&acc[c], (double)xx[e]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                qc[d] = nc[d];
+                qv[d] = nv[d];
+            }
             sb = nsb;
             if (sb >= s1) break;
         }
@@ -1903,10 +1948,13 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
             limn = vout[vn + 1];
             lon = vlo[vn];
             nwn = (vwid[vn] + 31) >> 5;
-            const long long nb =
-                min((s0n & ~3LL) + (long long)threadIdx.x * 4, (s1n - 1) & ~3LL);
-            qc = *(const int4*)(icol + nb);
-            qv = *(const float4*)(ival + nb);
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const long long nb = min((s0n & ~3LL) + (long long)(d * kPT + threadIdx.x) * 4,
+                                         (s1n - 1) & ~3LL);
+                qc[d] = *(const int4*)(icol + islot(nb));
+                qv[d] = *(const float4*)(ival + islot(nb));
+            }
         }
         const unsigned b = threadIdx.x < NW ? bits[threadIdx.x] : 0u;
         int tot;
@@ -1982,8 +2030,8 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
         // group is in flight while the current one is applied
         const long long glast = (s1 - 1) & ~3LL;
         long long ga = min((s0 & ~3LL) + (long long)lane * 4, glast);
-        int4 qc = *(const int4*)(icol + ga);
-        float4 qv = *(const float4*)(ival + ga);
+        int4 qc = *(const int4*)(icol + islot(ga));
+        float4 qv = *(const float4*)(ival + islot(ga));
         uint4 wq;
         if constexpr (BUILD) {
             *(uint4*)(bits + lane * 4) = make_uint4(0u, 0u, 0u, 0u);
@@ -1991,7 +2039,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
             __builtin_amdgcn_wave_barrier();
             for (long long sb = s0 & ~3LL; sb < s1; sb += 4 * 64) {
                 const long long base = sb + (long long)lane * 4;
-                const int4 q = *(const int4*)(icol + min(base, (s1 - 1) & ~3LL));
+                const int4 q = *(const int4*)(icol + islot(min(base, (s1 - 1) & ~3LL)));
                 const int cc[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
@@ -2018,7 +2066,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
         const int ex = inc - mine;
         if constexpr (BUILD) {
             const int total = __shfl(inc, 63, 64);
-            if (lane == 0) cnt_out[v] = total;
+            if (cnt_out && lane == 0) cnt_out[v] = total;
             cnt = min(cnt, total);  // the capacity may exceed the count
         } else {
             *(uint4*)(bits + lane * 4) = wq;
@@ -2037,8 +2085,8 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
             float4 nv = qv;
             if (nsb < s1) {  // uniform
                 ga = min(nsb + (long long)lane * 4, glast);
-                nc = *(const int4*)(icol + ga);
-                nv = *(const float4*)(ival + ga);
+                nc = *(const int4*)(icol + islot(ga));
+                nv = *(const float4*)(ival + islot(ga));
             }
             const int cc[4] = {qc.x, qc.y, qc.z, qc.w};
             const float xx[4] = {qv.x, qv.y, qv.z, qv.w};
@@ -2114,7 +2162,7 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
                 int c[4];
     #pragma unroll
                 for (int u = 0; u < 4; ++u)
-                    c[u] = icol[min(sb + threadIdx.x + (long long)u * kPT, s1 - 1)];
+                    c[u] = icol[islot(min(sb + threadIdx.x + (long long)u * kPT, s1 - 1))];
     #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const long long q = sb + threadIdx.x + (long long)u * kPT;
@@ -2138,7 +2186,7 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
                 float x[4];
     #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const long long q = min(s + (long long)u * kPT, s1 - 1);
+                    const long long q = islot(min(s + (long long)u * kPT, s1 - 1));
                     c[u] = icol[q];
                     x[u] = ival[q];
                 }
@@ -2214,7 +2262,7 @@ __device__ __forceinline__ void rank_wave_load(const int* __restrict__ icol,
 #pragma unroll
     for (int u = 0; u < kRankWavePer; ++u) {
         long long q = s0 + u * 64 + lane;
-        q = q < s1 ? q : last;
+        q = islot(q < s1 ? q : last);
         c[u] = icol[q];
         x[u] = ival[q];
     }
@@ -2290,7 +2338,7 @@ __global__ __launch_bounds__(64) void k_merge_rank_wave(
                 }
             }
             for (long long q = s0 + PER * 64 + lane; q < s1; q += 64) {
-                const int cc = icol[q] - lo;
+                const int cc = icol[islot(q)] - lo;
                 atomicOr(&bits[cc >> 5], 1u << (cc & 31));
             }
             for (int q = lane; q < cnt; q += 64) acc[q] = 0.0;
@@ -2341,10 +2389,10 @@ __global__ __launch_bounds__(64) void k_merge_rank_wave(
                 }
             }
             for (long long q = s0 + PER * 64 + lane; q < s1; q += 64) {
-                const int cc = icol[q] - lo;
+                const int cc = icol[islot(q)] - lo;
                 const int r0 = rank_of(bits, pre, cc);
                 if (r0 < cnt) {
-                    unsafeAtomicAdd(&acc[r0], (double)ival[q]);
+                    unsafeAtomicAdd(&acc[r0], (double)ival[islot(q)]);
                     colr[r0] = cc + lo;
                 }
             }
@@ -2382,8 +2430,8 @@ struct MergeOut {
 template <int LOG2H>
 static void launch_merge_hash(spg_context* c, const int* list, long long cnt, const MergeOut& o) {
     hipLaunchKernelGGL(k_merge_hash<LOG2H>, dim3((unsigned)cnt), dim3(kBlock), 0, c->stream,
-                       list, (const long long*)c->vbase.p, (const int*)c->icol.p,
-                       (const float*)c->ival.p, o.obase, o.ocol, o.oval, o.cnt);
+                       list, (const long long*)c->vbase.p, (const int*)inter_cols(c),
+                       (const float*)inter_vals(c), o.obase, o.ocol, o.oval, o.cnt);
 }
 
 // One merge kernel over the virtual rows list[0..cnt) of class b.  build:
@@ -2392,8 +2440,8 @@ static int launch_merge_class(spg_context* c, int b, const int* list, long long 
                               const MergeOut& o, bool build) {
     const long long* vbase = (const long long*)c->vbase.p;
     const int* vlo = (const int*)c->vlo.p;
-    const int* icol = (const int*)c->icol.p;
-    const float* ival = (const float*)c->ival.p;
+    const int* icol = inter_cols(c);
+    const float* ival = inter_vals(c);
     const unsigned* tb = (const unsigned*)c->tilebits.p;
     if (b == kClassTileS) {
         const dim3 g(persistent_grid(c, (cnt + 7) / 8, 4));
@@ -2488,7 +2536,8 @@ int launch_merge(spg_context* c) {
     tmark(c, SPG_PHASE_MERGE, 0);
     for (int b = kNumVClasses - 1; b >= 0; --b) {
         const long long cnt = off[b + 1] - off[b];
-        if (cnt > 0) SPG_TRY(launch_merge_class(c, b, vl + off[b], cnt, o, false));
+        // small tile rows build their bitmap (the symbolic pass kept none)
+        if (cnt > 0) SPG_TRY(launch_merge_class(c, b, vl + off[b], cnt, o, b == kClassTileS));
     }
     tmark(c, SPG_PHASE_MERGE, 1);
     return SPG_OK;

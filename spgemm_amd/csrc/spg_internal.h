@@ -83,8 +83,10 @@ struct spg_context {
     spg::DevBuf kbin, kperm, kbin_off, kitems, kitem_off;
     int64_t kbin_off_h[SPG_NUM_KBINS + 1] = {};
     int64_t kitems_bin_h[SPG_NUM_KBINS + 1] = {};  // item offsets at bin starts
-    // intermediates
-    spg::DevBuf icol, ival;
+    // intermediates, in groups of 16 products: each 128-byte group holds the
+    // 16 columns, then the 16 values (product s: word ((s >> 4) << 5) + (s & 15)
+    // of the column half, 16 words later its value)
+    spg::DevBuf inter;
     // symbolic / merge
     spg::DevBuf vclass, vlist, vclass_off, vcnt, vout, crow, ccol, cval;
     // learned merge: row predictions, per-virtual-row capacities, padded
@@ -122,6 +124,26 @@ struct spg_context {
     long long graph_pred_version = -1;
     spg_learned_stats graph_stats{};
 };
+
+#ifndef SPG_IGROUP_LOG2
+#define SPG_IGROUP_LOG2 4
+#endif
+namespace spg {
+constexpr int kIGroupLog2 = SPG_IGROUP_LOG2;  // products per group (log2)
+constexpr long long kIGroup = 1LL << kIGroupLog2;
+// Word index of intermediate product s in the grouped layout (its column in
+// inter_cols, its value at the same index of inter_vals).
+__host__ __device__ __forceinline__ long long islot(long long s) {
+    return ((s >> kIGroupLog2) << (kIGroupLog2 + 1)) + (s & (kIGroup - 1));
+}
+// Words of the grouped buffer for np products (whole groups, one spare
+// group: 16-byte loads of a row's last group stay inside).
+inline size_t inter_words(long long np) {
+    return (size_t)((np + kIGroup - 1) / kIGroup + 1) * (size_t)(2 * kIGroup);
+}
+inline int* inter_cols(spg_context* c) { return (int*)c->inter.p; }
+inline float* inter_vals(spg_context* c) { return (float*)c->inter.p + kIGroup; }
+}  // namespace spg
 
 namespace spg {
 
