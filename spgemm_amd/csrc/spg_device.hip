@@ -25,6 +25,7 @@
 #include "spg_internal.h"
 
 #include <algorithm>
+#include <climits>
 
 namespace spg {
 
@@ -398,6 +399,16 @@ __device__ __forceinline__ T wave_max(T x) {
     for (int o = 32; o > 0; o >>= 1) {
         const T y = __shfl_xor(x, o, 64);
         x = y > x ? y : x;
+    }
+    return x;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_min(T x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const T y = __shfl_xor(x, o, 64);
+        x = y < x ? y : x;
     }
     return x;
 }
@@ -2135,7 +2146,12 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
     __shared__ double acc[kHashMaxCount];
     __shared__ int sh[kPT / 64];
     __shared__ long long sj;
+    __shared__ int smin, smax;  // the row's smallest and largest window column
     for (int w = threadIdx.x; w < kRankMaxWords; w += kPT) bits[w] = 0u;
+    if (threadIdx.x == 0) {
+        smin = INT_MAX;
+        smax = -1;
+    }
     // rows in chunks of kRankChunk from a global counter (few atomics, and
     // rows of uneven size still balance)
     for (;;) {
@@ -2149,11 +2165,7 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
             const long long o = vout[v];
             const int cnt = (int)min((long long)kHashMaxCount, vout[v + 1] - o);
             const int lo = vlo[v];                  // columns of the row: [lo, lo + width)
-            const int nw = (vwid[v] + 31) >> 5;     // width <= 2^18 (class condition)
-            // an even number of bitmap words per thread, enough for the window
-            const int wpt = (((nw + kPT - 1) / kPT) + 1) & ~1;
-            const int w0 = threadIdx.x * wpt;
-            const int w1 = min(w0 + wpt, nw);
+            int cmin = INT_MAX, cmax = -1;
             // trip count uniform across the workgroup (the OR below shuffles);
             // positions past the row read its last product (rows of the
             // class are non-empty) and are masked after the loads, so the
@@ -2166,11 +2178,30 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
     #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const long long q = sb + threadIdx.x + (long long)u * kPT;
-                    wave_bitmap_or(bits, q < s1 ? c[u] - lo : -1);
+                    const int cu = q < s1 ? c[u] - lo : -1;
+                    wave_bitmap_or(bits, cu);
+                    if (cu >= 0) {
+                        cmin = min(cmin, cu);
+                        cmax = max(cmax, cu);
+                    }
                 }
+            }
+            cmin = wave_min(cmin);
+            cmax = wave_max(cmax);
+            if (lane_id() == 0) {
+                atomicMin(&smin, cmin);
+                atomicMax(&smax, cmax);
             }
             for (int q = threadIdx.x; q < cnt; q += kPT) acc[q] = 0.0;
             __syncthreads();
+            // only the words of the row's column span [wlo, whi) are scanned,
+            // emitted and cleared (a row of few columns in a wide window
+            // costs its span, not the window); an even number of words per
+            // thread, pairs aligned
+            const int wlo = (smin >> 5) & ~1, whi = (smax >> 5) + 1;
+            const int wpt = (((whi - wlo + kPT - 1) / kPT) + 1) & ~1;
+            const int w0 = wlo + threadIdx.x * wpt;
+            const int w1 = min(w0 + wpt, whi);
             int mine = 0;
             for (int w = w0; w < w1; ++w) mine += __popc(bits[w]);
             int tot;
@@ -2178,7 +2209,7 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
             if (cnt_out && threadIdx.x == 0) cnt_out[v] = tot;
             for (int w = w0; w < w1; w += 2) {
                 pre[w >> 1] = (unsigned short)run;
-                run += __popc(bits[w]) + __popc(bits[w + 1]);  // bits past nw are zero
+                run += __popc(bits[w]) + __popc(bits[w + 1]);  // bits past whi are zero
             }
             __syncthreads();
             for (long long s = s0 + threadIdx.x; s < s1; s += 4 * kPT) {
@@ -2205,12 +2236,16 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
             // emit: word pairs interleaved over the threads (neighbouring
             // threads read neighbouring accumulators and write neighbouring
             // output positions); a thread clears the pair it emitted
-            for (int pw = threadIdx.x; 2 * pw < nw; pw += kPT) {
+            if (threadIdx.x == 0) {  // every thread has read the span (before the scan's barriers)
+                smin = INT_MAX;
+                smax = -1;
+            }
+            for (int pw = (wlo >> 1) + threadIdx.x; 2 * pw < whi; pw += kPT) {
                 int r = pre[pw];
     #pragma unroll
                 for (int h = 0; h < 2; ++h) {
                     const int w = 2 * pw + h;
-                    unsigned b = bits[w];  // words past nw are zero
+                    unsigned b = bits[w];  // words past whi are zero
                     while (b) {
                         const int bit = __ffs(b) - 1;
                         b &= b - 1;

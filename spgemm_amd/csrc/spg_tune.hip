@@ -8,7 +8,8 @@
 // one tunable is timed (wall clock of a full product or batch, median of
 // reps runs after one warm-up) with the others held, the fastest kept if it
 // beats the incumbent by more than 1 %, and the tunables are visited in an
-// order drawn from the seed, twice.  The winner is set on the context and
+// order drawn from the seed; then every pair of long_row_ub and
+// rank_min_count (they interact) is timed, and a last sweep follows.  The winner is set on the context and
 // can be appended to a table (one JSON object per line) keyed by the
 // workload's shape, which spgTuneLookup applies later.
 #include <algorithm>
@@ -114,7 +115,7 @@ int spgTune(spg_context* c, const spg_predictor* p, int reps, uint64_t seed,
     std::vector<int> order = {0, 1, 2, 3, 4, 5};
     Xoshiro rng(seed);
     for (int q = 5; q > 0; --q) std::swap(order[(size_t)q], order[(size_t)rng.below((uint64_t)q + 1)]);
-    for (int sweep = 0; sweep < 2; ++sweep) {
+    auto sweep_once = [&]() {
         for (int d : order) {
             const long long cur = get(inc, d);
             for (long long v : dims[(size_t)d].values) {
@@ -128,7 +129,27 @@ int spgTune(spg_context* c, const spg_predictor* p, int reps, uint64_t seed,
                 }
             }
         }
+    };
+    sweep_once();
+    // long_row_ub and rank_min_count interact (rows split into tiles never
+    // reach the wide-row classes), so one coordinate at a time can stall:
+    // every pair of the two is timed once, then a last sweep
+    {
+        const spg_config base = inc;
+        for (long long lu : dims[1].values)
+            for (long long rm : dims[5].values) {
+                if (lu == base.long_row_ub && rm == base.rank_min_count) continue;
+                spg_config cand = base;
+                set(cand, 1, lu);
+                set(cand, 5, rm);
+                const double ms = time_config(c, p, cand, reps);
+                if (ms >= 0 && ms < inc_ms * 0.99) {
+                    inc = cand;
+                    inc_ms = ms;
+                }
+            }
     }
+    sweep_once();
     inc.timing = timing;
     SPG_TRY(spgSetConfig(c, &inc));
     if (best) *best = inc;
