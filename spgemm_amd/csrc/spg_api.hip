@@ -418,7 +418,11 @@ int spgInitGEMM(spg_context* c, cm* C, int64_t* nprod) {
         return fail(c, SPG_EINVAL, "B has more than 1024 column tiles; raise tile_log2");
     for (auto& r : c->timer.recorded) r = false;
     tmark(c, SPG_PHASE_INIT, 0);
-    SPG_TRY(scratch_prepare(c, std::max<long long>(c->m, c->K) + 1));
+    {
+        // the transpose of A (scratch_prepare gives 4 MB + 16 B per element)
+        const long long tr = (long long)(transpose_scratch(c->m, c->A.nnz) / 16) + 1;
+        SPG_TRY(scratch_prepare(c, std::max<long long>(std::max<long long>(c->m, c->K) + 1, tr)));
+    }
     SPG_TRY(ensure(c, c->ub, sizeof(int64_t) * ((size_t)c->m + 1)));
     SPG_TRY(ensure(c, c->ntile, sizeof(int32_t) * ((size_t)c->m + 1)));
     SPG_TRY(ensure(c, c->vstart, sizeof(int32_t) * ((size_t)c->m + 1)));

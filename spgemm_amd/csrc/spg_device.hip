@@ -608,10 +608,205 @@ __global__ __launch_bounds__(kBlock) void k_row_tile_bound(const int* __restrict
     }
 }
 
+// A transposed into rows without global atomics (a counting sort in two
+// levels, all counters in LDS).  Random global atomics run at a fixed rate
+// (about 27e9/s on this part, whatever their scope), so one per entry, as
+// the row histogram and fill below do, costs 2 ms per 53M entries.  Here:
+// (0) the column of every entry; (1) per tile of E entries, an LDS
+// histogram of row buckets (2^BL consecutive rows, at most 4096 buckets),
+// scanned bucket-major over the tiles, and (2) the tile's entries scattered
+// to their bucket's range; (3) one workgroup per bucket counts its rows in
+// LDS, writes their row pointers (bucket start + prefix) and places every
+// entry.  The order of the entries inside a row is arbitrary, as with the
+// atomic fill.
+constexpr int kTrMaxBuckets = 4096;
+constexpr int kTrMaxBucketLog2 = 14;  // rows per bucket <= 16384 (64 KB of LDS counters)
+
+__global__ __launch_bounds__(kBlock) void k_tr_kidx(const int* __restrict__ a_ptr, int K,
+                                                     int* __restrict__ kidx) {
+    const int lane = lane_id(), sub = lane & (kGroup - 1);
+    const long long k = ((long long)blockIdx.x * kBlock + threadIdx.x) / kGroup;
+    int e0 = 0, e1 = 0;
+    if (k < K) {
+        e0 = a_ptr[k];
+        e1 = a_ptr[k + 1];
+    }
+    const bool lng = e1 - e0 > kShortSeg;
+    if (!lng)
+        for (int e = e0 + sub; e < e1; e += kGroup) kidx[e] = (int)k;
+    unsigned long long mask = __ballot(lng && sub == 0);
+    while (mask) {
+        const int l = __ffsll((long long)mask) - 1;
+        mask &= mask - 1;
+        const int b0 = __shfl(e0, l, 64), b1 = __shfl(e1, l, 64);
+        const int kk = (int)(((long long)blockIdx.x * kBlock + (threadIdx.x - lane + l)) / kGroup);
+        for (int e = b0 + lane; e < b1; e += 64) kidx[e] = kk;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_tr_count(const int* __restrict__ a_idx,
+                                                      long long nnz, int E, int BL, int NB,
+                                                      long long ntiles, int* __restrict__ H) {
+    __shared__ int hist[kTrMaxBuckets];
+    for (int b = threadIdx.x; b < NB; b += kBlock) hist[b] = 0;
+    __syncthreads();
+    const long long e0 = (long long)blockIdx.x * E, e1 = min(e0 + E, nnz);
+    for (long long e = e0 + threadIdx.x; e < e1; e += 4 * kBlock) {
+        int r[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) r[u] = a_idx[min(e + u * kBlock, e1 - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (e + u * kBlock < e1) atomicAdd(&hist[r[u] >> BL], 1);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < NB; b += kBlock) H[(long long)b * ntiles + blockIdx.x] = hist[b];
+}
+
+__global__ __launch_bounds__(kBlock) void k_tr_scatter(
+    const int* __restrict__ a_idx, const float* __restrict__ a_val, const int* __restrict__ kidx,
+    long long nnz, int E, int BL, int NB, long long ntiles, const int* __restrict__ O,
+    int* __restrict__ tr, int* __restrict__ tk, float* __restrict__ tv) {
+    __shared__ int cur[kTrMaxBuckets];
+    for (int b = threadIdx.x; b < NB; b += kBlock) cur[b] = O[(long long)b * ntiles + blockIdx.x];
+    __syncthreads();
+    const long long e0 = (long long)blockIdx.x * E, e1 = min(e0 + E, nnz);
+    for (long long e = e0 + threadIdx.x; e < e1; e += 4 * kBlock) {
+        int r[4], k[4];
+        float x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const long long q = min(e + u * kBlock, e1 - 1);
+            r[u] = a_idx[q];
+            k[u] = kidx[q];
+            x[u] = a_val[q];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (e + u * kBlock < e1) {
+                const int pos = atomicAdd(&cur[r[u] >> BL], 1);
+                tr[pos] = r[u];
+                tk[pos] = k[u];
+                tv[pos] = x[u];
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_tr_bucket(
+    const int* __restrict__ tr, const int* __restrict__ tk, const float* __restrict__ tv,
+    const int* __restrict__ O, long long ntiles, int NB, int BL, int m, int* __restrict__ rptr,
+    int* __restrict__ rk, float* __restrict__ rv) {
+    __shared__ int rc[1 << kTrMaxBucketLog2];
+    __shared__ int sh[kBlock / 64];
+    const int b = blockIdx.x;
+    const int nrow = 1 << BL, r0 = b << BL;
+    const int p0 = O[(long long)b * ntiles], p1 = O[(long long)(b + 1) * ntiles];
+    for (int i = threadIdx.x; i < nrow; i += kBlock) rc[i] = 0;
+    __syncthreads();
+    for (int p = p0 + threadIdx.x; p < p1; p += kBlock) atomicAdd(&rc[tr[p] - r0], 1);
+    __syncthreads();
+    // exclusive scan of the bucket's row counts: consecutive counters per thread
+    const int per = (nrow + kBlock - 1) / kBlock;
+    const int i0 = min(nrow, threadIdx.x * per), i1 = min(nrow, i0 + per);
+    int mine = 0;
+    for (int i = i0; i < i1; ++i) mine += rc[i];
+    int tot;
+    int run = block_exscan<int>(mine, &tot, sh);
+    for (int i = i0; i < i1; ++i) {
+        const int cnt = rc[i];
+        rc[i] = run;  // the row's cursor (relative to the bucket)
+        if (r0 + i < m) rptr[r0 + i] = p0 + run;
+        run += cnt;
+    }
+    if (b == NB - 1 && threadIdx.x == 0) rptr[m] = p1;
+    __syncthreads();
+    for (int p = p0 + threadIdx.x; p < p1; p += kBlock) {
+        const int pos = atomicAdd(&rc[tr[p] - r0], 1);
+        rk[p0 + pos] = tk[p];
+        rv[p0 + pos] = tv[p];
+    }
+}
+
+// Bucket parameters of the transpose, or false when the rows are too many
+// (more than 4096 buckets of 16384) and the atomic transpose is used.
+static bool tr_params(int m, long long nnz, int* BL, int* NB, int* E, long long* ntiles) {
+    int lg = 0;
+    while ((1LL << lg) < (long long)std::max(m, 1)) ++lg;
+    const int bl = std::max(0, lg - 12);
+    if (bl > kTrMaxBucketLog2) return false;
+    *BL = bl;
+    *NB = (int)(((long long)std::max(m, 1) + (1LL << bl) - 1) >> bl);
+    long long e = 16384;
+    if ((nnz + e - 1) / e > 4096) e = (((nnz + 4095) / 4096) + kBlock - 1) / kBlock * kBlock;
+    *E = (int)e;
+    *ntiles = std::max(1LL, (nnz + e - 1) / e);
+    return true;
+}
+
+// Scratch the transpose needs (bytes), 0 when it is not used.
+size_t transpose_scratch(int m, long long nnz) {
+    int BL, NB, E;
+    long long nt;
+    if (nnz <= 0 || !tr_params(m, nnz, &BL, &NB, &E, &nt)) return 0;
+    return (size_t)16 * (size_t)nnz + (size_t)8 * (size_t)NB * (size_t)nt + ((size_t)8 << 20);
+}
+
+// Upper bounds, tiling and features of every row (A by rows is built).
+static int launch_row_bound(spg_context* c) {
+    const int m = c->m;
+    if (m > 0) {
+        const long long nbm = ((long long)m * kGroup + kBlock - 1) / kBlock;  // 8 lanes per row
+        hipLaunchKernelGGL(k_row_bound, dim3((unsigned)nbm), dim3(kBlock), 0, c->stream,
+                           (const int*)c->arow_ptr.p, (const int*)c->arow_k.p, c->B.ptr, m,
+                           (long long)c->cfg.long_row_ub, c->T, (const int2*)c->rwin,
+                           c->cfg.tile_log2, (long long*)c->ub.p, (int*)c->ntile.p,
+                           (float4*)c->feat.p);
+        SPG_HIP(hipGetLastError());
+    }
+    return scan_i32(c, (const int32_t*)c->ntile.p, (int32_t*)c->vstart.p, m);
+}
+
+static int launch_transpose(spg_context* c, bool* done) {
+    *done = false;
+    const int m = c->m, K = c->K;
+    const long long nnz = c->A.nnz;
+    int BL, NB, E;
+    long long nt;
+    if (nnz <= 0 || m <= 0 || !tr_params(m, nnz, &BL, &NB, &E, &nt)) return SPG_OK;
+    int* kidx = (int*)scratch_alloc(c, sizeof(int) * (size_t)nnz);
+    int* tr = (int*)scratch_alloc(c, sizeof(int) * (size_t)nnz);
+    int* tk = (int*)scratch_alloc(c, sizeof(int) * (size_t)nnz);
+    float* tv = (float*)scratch_alloc(c, sizeof(float) * (size_t)nnz);
+    int* H = (int*)scratch_alloc(c, sizeof(int) * (size_t)NB * (size_t)nt);
+    int* O = (int*)scratch_alloc(c, sizeof(int) * ((size_t)NB * (size_t)nt + 1));
+    if (!kidx || !tr || !tk || !tv || !H || !O) return fail(c, SPG_ENOMEM, "transpose scratch");
+    const long long nbk = ((long long)K * kGroup + kBlock - 1) / kBlock;  // 8 lanes per column
+    hipLaunchKernelGGL(k_tr_kidx, dim3((unsigned)nbk), dim3(kBlock), 0, c->stream, c->A.ptr, K,
+                       kidx);
+    SPG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_tr_count, dim3((unsigned)nt), dim3(kBlock), 0, c->stream, c->A.idx, nnz,
+                       E, BL, NB, nt, H);
+    SPG_HIP(hipGetLastError());
+    SPG_TRY(scan_i32(c, H, O, (long long)NB * nt));
+    hipLaunchKernelGGL(k_tr_scatter, dim3((unsigned)nt), dim3(kBlock), 0, c->stream, c->A.idx,
+                       c->A.val, (const int*)kidx, nnz, E, BL, NB, nt, (const int*)O, tr, tk, tv);
+    SPG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_tr_bucket, dim3((unsigned)NB), dim3(kBlock), 0, c->stream,
+                       (const int*)tr, (const int*)tk, (const float*)tv, (const int*)O, nt, NB, BL,
+                       m, (int*)c->arow_ptr.p, (int*)c->arow_k.p, (float*)c->arow_v.p);
+    SPG_HIP(hipGetLastError());
+    *done = true;
+    return SPG_OK;
+}
+
 int launch_init(spg_context* c) {
     const int m = c->m, K = c->K;
     int* rptr = (int*)c->arow_ptr.p;
     int* rk = (int*)c->arow_k.p;
+    bool transposed = false;
+    SPG_TRY(launch_transpose(c, &transposed));
+    if (transposed) return launch_row_bound(c);
     int* rcnt = (int*)scratch_alloc(c, sizeof(int) * ((size_t)m + 1));
     if (!rcnt) return fail(c, SPG_ENOMEM, "init scratch");
     SPG_TRY(fill<int>(c, rcnt, 0, m + 1));
@@ -631,16 +826,7 @@ int launch_init(spg_context* c) {
                            c->A.idx, c->A.val, K, rcnt, rk, (float*)c->arow_v.p);
         SPG_HIP(hipGetLastError());
     }
-    if (m > 0) {
-        const long long nbm = ((long long)m * kGroup + kBlock - 1) / kBlock;  // 8 lanes per row
-        hipLaunchKernelGGL(k_row_bound, dim3((unsigned)nbm), dim3(kBlock), 0, c->stream,
-                           (const int*)rptr, (const int*)rk, c->B.ptr, m,
-                           (long long)c->cfg.long_row_ub, c->T, (const int2*)c->rwin,
-                           c->cfg.tile_log2, (long long*)c->ub.p, (int*)c->ntile.p,
-                           (float4*)c->feat.p);
-        SPG_HIP(hipGetLastError());
-    }
-    return scan_i32(c, (const int32_t*)c->ntile.p, (int32_t*)c->vstart.p, m);
+    return launch_row_bound(c);
 }
 
 int launch_init_vrows(spg_context* c) {
