@@ -183,7 +183,15 @@ typedef struct spg_config {
     int32_t rank_min_count;    /* wide rows with more distinct columns than this
                                   merge through a bitmap of their window (rank),
                                   smaller ones through an LDS hash               */
+    int32_t transpose_mode;    /* InitGEMM's A-by-rows transpose: SPG_TRANSPOSE_SORT
+                                  (counting sort in LDS, no global atomics; rows
+                                  past 2^26 use the atomic one) or
+                                  SPG_TRANSPOSE_ATOMIC (one atomic per entry, as
+                                  src/mm.cu:138-148 counts rows)               */
 } spg_config;
+
+#define SPG_TRANSPOSE_SORT 0
+#define SPG_TRANSPOSE_ATOMIC 1
 
 /* Expansion mappings of BinGEMM.  KBINS groups the (i,k) pairs by k as the
  * reference does (one work item per k-chunk, k binned by nnz(B row k);

@@ -35,6 +35,8 @@ PHASES = ("init", "categorize", "expand", "symbolic", "merge", "predict", "compa
 MTX_COMPAT_VALUES = 1
 EXPAND_KBINS = 0
 EXPAND_ROWS = 1
+TRANSPOSE_SORT = 0
+TRANSPOSE_ATOMIC = 1
 BATCH_GRAPH = 1
 
 STATUS = {0: "SPG_OK", 1: "SPG_EINVAL", 2: "SPG_ENOMEM", 3: "SPG_EHIP", 4: "SPG_EORDER",
@@ -70,7 +72,8 @@ class Config(Structure):
     _fields_ = [("tile_log2", c_int32), ("item_products", c_int32), ("long_row_ub", c_int64),
                 ("xcd_swizzle", c_int32), ("timing", c_int32), ("expand_mode", c_int32),
                 ("pred_margin", c_float), ("pred_slack", c_int32), ("pred_tight", c_int32),
-                ("heavy_factor", c_int32), ("rank_min_count", c_int32)]
+                ("heavy_factor", c_int32), ("rank_min_count", c_int32),
+                ("transpose_mode", c_int32)]
 
 
 class LearnedStats(Structure):

@@ -773,7 +773,9 @@ static int launch_transpose(spg_context* c, bool* done) {
     const long long nnz = c->A.nnz;
     int BL, NB, E;
     long long nt;
-    if (nnz <= 0 || m <= 0 || !tr_params(m, nnz, &BL, &NB, &E, &nt)) return SPG_OK;
+    if (c->cfg.transpose_mode != SPG_TRANSPOSE_SORT || nnz <= 0 || m <= 0 ||
+        !tr_params(m, nnz, &BL, &NB, &E, &nt))
+        return SPG_OK;
     int* kidx = (int*)scratch_alloc(c, sizeof(int) * (size_t)nnz);
     int* tr = (int*)scratch_alloc(c, sizeof(int) * (size_t)nnz);
     int* tk = (int*)scratch_alloc(c, sizeof(int) * (size_t)nnz);

@@ -163,6 +163,30 @@ def test_long_row_tiling(spg, orc, gpu_ctx, tile_log2, long_ub, mode):
     compare(out, oracle_all(orc, A, sh, A, sh))
 
 
+@pytest.mark.parametrize("tmode", [0, 1], ids=["sort", "atomic"])
+def test_transpose_modes(spg, orc, gpu_ctx, tmode):
+    """InitGEMM's A-by-rows transpose: the LDS counting sort (default) and
+    the one-atomic-per-entry transpose give the same bounds, intermediates
+    and C, on a power-law product (hub rows in one bucket) and on rows with
+    empty buckets."""
+    print(f"case transpose mode={tmode}", flush=True)
+    A, sh = gen(spg, "rmat", s=13, ef=16, seed=31)
+    out = run_gpu(spg, gpu_ctx, A, sh, A, sh,
+                  cfg=dict(transpose_mode=tmode, expand_mode=spg.EXPAND_ROWS))
+    compare(out, oracle_all(orc, A, sh, A, sh))
+    # rows far apart: most row buckets empty
+    m, K = 70000, 300
+    rng = np.random.default_rng(5)
+    rows = np.sort(rng.choice(m, 900, replace=False)).astype(np.int32)
+    cols = rng.integers(0, K, 900).astype(np.int32)
+    vals = rng.uniform(-1, 1, 900).astype(np.float32)
+    Bm = spg.gen_uniform_rows(K, 500, 6, 32)
+    B, Bsh = (Bm.rows, Bm.cols, Bm.vals), (Bm.num_rows, Bm.num_cols)
+    out = run_gpu(spg, gpu_ctx, (rows, cols, vals), (m, K), B, Bsh,
+                  cfg=dict(transpose_mode=tmode, expand_mode=spg.EXPAND_ROWS))
+    compare(out, oracle_all(orc, (rows, cols, vals), (m, K), B, Bsh))
+
+
 def test_simple_equals_binned(spg, orc, gpu_ctx):
     print("case simple vs binned", flush=True)
     A, sh = gen(spg, "rmat", s=12, ef=8, seed=3)
