@@ -957,6 +957,10 @@ __device__ __forceinline__ long long next_item(unsigned long long* ctr, long lon
 // consecutive slots.  A group of G lanes (G = next power of two >= the B row
 // piece, at most 64) serves one A entry, so short B rows do not idle lanes.
 constexpr int kTBCap = 2048;     // LDS slot bases of long-row tiles per batch
+// Workgroups of the binned expansion per CU: its LDS (24.6 KB) allows six,
+// and its registers are held to six waves per SIMD (measured: 6 beats 5, 7
+// and 8, which spill or shrink the slot tables)
+constexpr int kExpandWaves = 6;
 constexpr int kItemChunk = 8;    // work items taken per counter increment
 
 struct ExpandArgs {
@@ -1131,7 +1135,7 @@ __device__ __forceinline__ int lanes_log2(int nj, long long ne) {
 // bin first).  With the XCD mapping on, chunk c is served by the XCD c % 8
 // so the items of one chunk (mostly one k) share that XCD's L2 copy of B
 // row k.
-__global__ __launch_bounds__(kBlock) void k_expand(ExpandArgs g,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandWaves))) void k_expand(ExpandArgs g,
                                                     unsigned long long* __restrict__ ctr) {
     __shared__ ExpandLds L;
     __shared__ long long sj;
@@ -1223,7 +1227,7 @@ int launch_expand_binned(spg_context* c) {
     unsigned long long* ctr = (unsigned long long*)c->counters.p + 8;
     SPG_TRY(fill<unsigned long long>(c, ctr, 0ull, 8));
     const long long nchunks = (g.nitems + kItemChunk - 1) / kItemChunk;
-    unsigned grid = persistent_grid(c, nchunks, 6);
+    unsigned grid = persistent_grid(c, nchunks, kExpandWaves);
     if (g.swizzle) grid = (grid + 7) & ~7u;  // every XCD part gets workgroups
     hipLaunchKernelGGL(k_expand, dim3(grid), dim3(kBlock), 0, c->stream, g, ctr);
     SPG_HIP(hipGetLastError());
@@ -2675,12 +2679,14 @@ static int launch_merge_class(spg_context* c, int b, const int* list, long long 
             hipLaunchKernelGGL((k_merge_tile_wave<256, 8, false>), g, dim3(512), 0, c->stream, list,
                                cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval, o.cnt);
     } else if (b == kClassTileM) {
-        const dim3 g(persistent_grid(c, (cnt + 3) / 4, 2));
+        // one wave per workgroup: 13.8 KB of LDS each, so a CU holds 11 of
+        // them (four-wave workgroups fit only twice: 8 waves)
+        const dim3 g(persistent_grid(c, cnt, 11));
         if (build)
-            hipLaunchKernelGGL((k_merge_tile_wave<1024, 4, true>), g, dim3(256), 0, c->stream, list,
+            hipLaunchKernelGGL((k_merge_tile_wave<1024, 1, true>), g, dim3(64), 0, c->stream, list,
                                cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval, o.cnt);
         else
-            hipLaunchKernelGGL((k_merge_tile_wave<1024, 4, false>), g, dim3(256), 0, c->stream,
+            hipLaunchKernelGGL((k_merge_tile_wave<1024, 1, false>), g, dim3(64), 0, c->stream,
                                list, cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval,
                                o.cnt);
     } else if (b == kClassTile) {
