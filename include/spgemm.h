@@ -184,14 +184,17 @@ typedef struct spg_config {
                                   merge through a bitmap of their window (rank),
                                   smaller ones through an LDS hash               */
     int32_t transpose_mode;    /* InitGEMM's A-by-rows transpose: SPG_TRANSPOSE_SORT
-                                  (counting sort in LDS, no global atomics; rows
-                                  past 2^26 use the atomic one) or
+                                  (counting sort in LDS, no global atomics; more
+                                  than 2^26 rows use the atomic one),
                                   SPG_TRANSPOSE_ATOMIC (one atomic per entry, as
-                                  src/mm.cu:138-148 counts rows)               */
+                                  src/mm.cu:138-148 counts rows) or
+                                  SPG_TRANSPOSE_AUTO (the sort from 2^20 entries
+                                  on, the atomic one below: fewer launches)     */
 } spg_config;
 
-#define SPG_TRANSPOSE_SORT 0
+#define SPG_TRANSPOSE_AUTO 0
 #define SPG_TRANSPOSE_ATOMIC 1
+#define SPG_TRANSPOSE_SORT 2
 
 /* Expansion mappings of BinGEMM.  KBINS groups the (i,k) pairs by k as the
  * reference does (one work item per k-chunk, k binned by nnz(B row k);

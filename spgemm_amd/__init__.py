@@ -35,8 +35,9 @@ PHASES = ("init", "categorize", "expand", "symbolic", "merge", "predict", "compa
 MTX_COMPAT_VALUES = 1
 EXPAND_KBINS = 0
 EXPAND_ROWS = 1
-TRANSPOSE_SORT = 0
+TRANSPOSE_AUTO = 0
 TRANSPOSE_ATOMIC = 1
+TRANSPOSE_SORT = 2
 BATCH_GRAPH = 1
 
 STATUS = {0: "SPG_OK", 1: "SPG_EINVAL", 2: "SPG_ENOMEM", 3: "SPG_EHIP", 4: "SPG_EORDER",

@@ -166,7 +166,7 @@ int spgCreate(spg_context** out, int device) {
     c->cfg.pred_slack = 8;
     c->cfg.heavy_factor = 50;
     c->cfg.rank_min_count = 128;
-    c->cfg.transpose_mode = SPG_TRANSPOSE_SORT;
+    c->cfg.transpose_mode = SPG_TRANSPOSE_AUTO;
     int rc = SPG_OK;
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -253,8 +253,8 @@ int spgSetConfig(spg_context* c, const spg_config* cfg) {
         return fail(c, SPG_EINVAL, "heavy_factor must be in [1, 2^20]");
     if (cfg->rank_min_count < 0 || cfg->rank_min_count > kHashMaxCount)
         return fail(c, SPG_EINVAL, "rank_min_count must be in [0, 4096]");
-    if (cfg->transpose_mode != SPG_TRANSPOSE_SORT && cfg->transpose_mode != SPG_TRANSPOSE_ATOMIC)
-        return fail(c, SPG_EINVAL, "transpose_mode must be SPG_TRANSPOSE_SORT or SPG_TRANSPOSE_ATOMIC");
+    if (cfg->transpose_mode < SPG_TRANSPOSE_AUTO || cfg->transpose_mode > SPG_TRANSPOSE_SORT)
+        return fail(c, SPG_EINVAL, "transpose_mode must be AUTO, ATOMIC or SORT");
     c->cfg = *cfg;
     c->stage = std::min(c->stage, 0);
     graph_release(c);
@@ -423,10 +423,7 @@ int spgInitGEMM(spg_context* c, cm* C, int64_t* nprod) {
     tmark(c, SPG_PHASE_INIT, 0);
     {
         // the transpose of A (scratch_prepare gives 4 MB + 16 B per element)
-        const long long tr =
-            c->cfg.transpose_mode == SPG_TRANSPOSE_SORT
-                ? (long long)(transpose_scratch(c->m, c->A.nnz) / 16) + 1
-                : 0;
+        const long long tr = (long long)(transpose_scratch(c, c->m, c->A.nnz) / 16) + 1;
         SPG_TRY(scratch_prepare(c, std::max<long long>(std::max<long long>(c->m, c->K) + 1, tr)));
     }
     SPG_TRY(ensure(c, c->ub, sizeof(int64_t) * ((size_t)c->m + 1)));

@@ -744,11 +744,19 @@ static bool tr_params(int m, long long nnz, int* BL, int* NB, int* E, long long*
     return true;
 }
 
+// Whether InitGEMM transposes A by the counting sort: SORT always (when
+// the rows allow it), AUTO from 2^20 entries on (below, the atomic
+// transpose's fewer launches win), never with ATOMIC.
+static bool use_sort(const spg_context* c, long long nnz) {
+    return c->cfg.transpose_mode == SPG_TRANSPOSE_SORT ||
+           (c->cfg.transpose_mode == SPG_TRANSPOSE_AUTO && nnz >= (1LL << 20));
+}
+
 // Scratch the transpose needs (bytes), 0 when it is not used.
-size_t transpose_scratch(int m, long long nnz) {
+size_t transpose_scratch(const spg_context* c, int m, long long nnz) {
     int BL, NB, E;
     long long nt;
-    if (nnz <= 0 || !tr_params(m, nnz, &BL, &NB, &E, &nt)) return 0;
+    if (!use_sort(c, nnz) || nnz <= 0 || !tr_params(m, nnz, &BL, &NB, &E, &nt)) return 0;
     return (size_t)16 * (size_t)nnz + (size_t)8 * (size_t)NB * (size_t)nt + ((size_t)8 << 20);
 }
 
@@ -773,8 +781,7 @@ static int launch_transpose(spg_context* c, bool* done) {
     const long long nnz = c->A.nnz;
     int BL, NB, E;
     long long nt;
-    if (c->cfg.transpose_mode != SPG_TRANSPOSE_SORT || nnz <= 0 || m <= 0 ||
-        !tr_params(m, nnz, &BL, &NB, &E, &nt))
+    if (!use_sort(c, nnz) || nnz <= 0 || m <= 0 || !tr_params(m, nnz, &BL, &NB, &E, &nt))
         return SPG_OK;
     int* kidx = (int*)scratch_alloc(c, sizeof(int) * (size_t)nnz);
     int* tr = (int*)scratch_alloc(c, sizeof(int) * (size_t)nnz);
@@ -1627,8 +1634,8 @@ constexpr int kSymWaves = kPT / 64;
 
 __global__ __launch_bounds__(kPT) void k_symbolic_tile(
     const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
-    const int* __restrict__ vlo, const int* __restrict__ vwid, const int* __restrict__ icol,
-    long long* __restrict__ vcnt, unsigned* __restrict__ tilebits) {
+    const int* __restrict__ vlo, const int* __restrict__ icol, long long* __restrict__ vcnt,
+    unsigned* __restrict__ tilebits) {
     __shared__ unsigned bitsall[kSymWaves][kTileWords];
     const int wv = threadIdx.x >> 6, lane = lane_id();
     unsigned* bits = bitsall[wv];
@@ -1720,7 +1727,6 @@ __global__ void k_rowptr(const int* __restrict__ vstart, const long long* __rest
 // Small wide rows (at most 2^(LOG2H-1) distinct columns, LOG2H <= 9): one
 // wave per row and a wave-private LDS hash, so many rows share a CU without
 // workgroup barriers.
-constexpr int kHashWaveMaxLog2 = 9;
 constexpr int kHashWaveWPB = 8;   // waves per workgroup (4 for 512 slots)
 
 template <int LOG2H, int WPB>
@@ -1877,7 +1883,7 @@ int launch_symbolic(spg_context* c) {
             hipLaunchKernelGGL(k_symbolic_tile,
                                dim3(persistent_grid(c, (cnt + kSymWaves - 1) / kSymWaves, 4)),
                                dim3(kPT), 0, c->stream, list, cnt, (const long long*)c->vbase.p,
-                               (const int*)c->vlo.p, (const int*)c->vwid.p,
+                               (const int*)c->vlo.p,
                                (const int*)inter_cols(c), (long long*)c->vcnt.p,
                                (unsigned*)c->tilebits.p);
         } else {
@@ -2329,10 +2335,10 @@ constexpr int kRankChunk = 8;  // rows taken per counter increment
 
 __global__ __launch_bounds__(kPT) void k_merge_rank(
     const int* __restrict__ vlist, long long nlist, unsigned long long* __restrict__ ctr,
-    const long long* __restrict__ vbase,
-    const int* __restrict__ vlo, const int* __restrict__ vwid, const int* __restrict__ icol,
-    const float* __restrict__ ival, const long long* __restrict__ vout,
-    int* __restrict__ ccol, float* __restrict__ cval, long long* __restrict__ cnt_out) {
+    const long long* __restrict__ vbase, const int* __restrict__ vlo,
+    const int* __restrict__ icol, const float* __restrict__ ival,
+    const long long* __restrict__ vout, int* __restrict__ ccol, float* __restrict__ cval,
+    long long* __restrict__ cnt_out) {
     __shared__ unsigned bits[kRankMaxWords];
     __shared__ unsigned short pre[kRankMaxWords / 2];  // rank before each word pair
     __shared__ double acc[kHashMaxCount];
@@ -2711,8 +2717,8 @@ static int launch_merge_class(spg_context* c, int b, const int* list, long long 
         hipLaunchKernelGGL(k_merge_rank,
                            dim3(persistent_grid(c, (cnt + kRankChunk - 1) / kRankChunk, 2)),
                            dim3(kPT), 0, c->stream, list, cnt,
-                           (unsigned long long*)c->counters.p + 2, vbase, vlo,
-                           (const int*)c->vwid.p, icol, ival, o.obase, o.ocol, o.oval, o.cnt);
+                           (unsigned long long*)c->counters.p + 2, vbase, vlo, icol, ival,
+                           o.obase, o.ocol, o.oval, o.cnt);
     } else {
         const dim3 gw(persistent_grid(c, (cnt + kHashWaveWPB - 1) / kHashWaveWPB, 4));
         const dim3 bw(64 * kHashWaveWPB);

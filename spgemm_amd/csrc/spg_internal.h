@@ -176,7 +176,7 @@ int stable_partition(spg_context* c, const int32_t* key, int64_t n, int nb,
 int launch_coo2cm(spg_context* c, const triplet* d_trip, int nnz, int type,
                   int nmajor, int* ptr, int* idx, float* val);
 int launch_init(spg_context* c);
-size_t transpose_scratch(int m, long long nnz);  // scratch bytes launch_init's transpose uses
+size_t transpose_scratch(const spg_context* c, int m, long long nnz);  // launch_init's transpose
 int launch_categorize(spg_context* c);
 int launch_expand_binned(spg_context* c);
 int launch_expand_simple(spg_context* c);
