@@ -1,0 +1,192 @@
+"""The benchmark workloads at their full sizes, checked through properties
+that do not need the oracle to run the whole product (it would take minutes
+to hours):
+
+* structure invariants: row pointers non-decreasing, columns in range and
+  strictly increasing inside every row, nnz(C_i) <= min(ub_i, n), and
+  sum(ub) = nprod = sum_k nnz(A col k) * nnz(B row k) (closed form on the
+  host);
+* closed forms where the workload has them: the 27-point stencil on 64^3 has
+  nprod = 566^3 and nnz(C) = 314^3 (every product of two 27-point stencils
+  reaches the 125-point neighbourhood clipped to the grid);
+* linearity (a checksum of every row): C x = A (B x) for a random x, in
+  float64 on the host, within VAL_RTOL of |A| (|B| |x|), the bound of the
+  value contract (FP32 products, FP64 sums, one FP32 rounding);
+* agreement of independent kernel paths on the same product: the learned
+  path (different merge kernels, capacities from the predictor, fallback,
+  compaction) gives the plain path's structure exactly and its values
+  within VAL_RTOL; the batched path gives every product's one-product
+  result.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+VAL_RTOL = 1e-5
+
+
+def csr_host(M, shape):
+    """A generated HostCOO as a float64 scipy CSR (duplicates summed: the
+    same linear map)."""
+    return sp.csr_matrix((M.vals.astype(np.float64), (M.rows, M.cols)), shape=shape)
+
+
+def check_structure(C, ub, n, nprod=None):
+    ptr, col, _ = C
+    m = len(ptr) - 1
+    assert ptr[0] == 0 and ptr[-1] == len(col)
+    cnt = np.diff(ptr)
+    assert np.all(cnt >= 0)
+    assert np.all(cnt <= np.minimum(ub, n))
+    if nprod is not None:
+        assert int(ub.sum()) == nprod
+    if len(col):
+        assert int(col.min()) >= 0 and int(col.max()) < n
+        first = np.zeros(len(col), bool)
+        first[ptr[:-1][cnt > 0]] = True  # the first entry of every non-empty row
+        assert np.all((col[1:] > col[:-1]) | first[1:])
+    return m
+
+
+def check_linear(C, A, B, n, seed):
+    ptr, col, val = C
+    m = len(ptr) - 1
+    x = np.random.default_rng(seed).uniform(-1.0, 1.0, n)
+    Cm = sp.csr_matrix((val.astype(np.float64), col, ptr), shape=(m, n))
+    y1 = Cm @ x
+    y2 = A @ (B @ x)
+    bound = abs(A) @ (abs(B) @ np.abs(x))
+    err = np.abs(y1 - y2)
+    assert np.all(err <= VAL_RTOL * bound + 1e-30), float((err / np.maximum(bound, 1e-300)).max())
+
+
+def same_c(C1, C2):
+    assert np.array_equal(C1[0], C2[0])
+    assert np.array_equal(C1[1], C2[1])
+    a, b = C1[2].astype(np.float64), C2[2].astype(np.float64)
+    err = np.abs(a - b) / np.maximum(np.abs(b), 1e-30)
+    err[(np.abs(a) <= 1e-30) & (np.abs(b) <= 1e-30)] = 0
+    assert err.max(initial=0.0) <= VAL_RTOL
+
+
+def nprod_of(A, B):
+    """sum_k nnz(A col k) * nnz(B row k), from the operands as the library
+    holds them (duplicates kept)."""
+    ka = np.bincount(A.cols, minlength=A.num_cols).astype(np.int64)
+    kb = np.bincount(B.rows, minlength=B.num_rows).astype(np.int64)
+    return int((ka * kb).sum())
+
+
+def predictor(spg, ctx):
+    from spgemm_amd import train_predictor as tp
+    P, _ = tp.load_or_train(ctx)
+    return P
+
+
+def test_rmat_s18_full(spg, gpu_ctx):
+    """R-MAT scale 18, edge factor 16: plain and learned paths."""
+    print("case full R-MAT s18 ef16", flush=True)
+    ctx = gpu_ctx
+    A = spg.gen_rmat(18, 18, 16 << 18, seed=0)
+    ctx.set_config(tile_log2=13, item_products=8192, long_row_ub=512, xcd_swizzle=1, timing=1,
+                   expand_mode=spg.EXPAND_KBINS, heavy_factor=50, rank_min_count=128)
+    ctx.set_operands_from_coo(A, A)
+    ctx.multiply()
+    ub = ctx.upper_bound(A.num_rows)
+    Cp = ctx.get_c()
+    check_structure(Cp, ub, A.num_cols, nprod_of(A, A))
+    Ah = csr_host(A, (A.num_rows, A.num_cols))
+    check_linear(Cp, Ah, Ah, A.num_cols, 1)
+    P = predictor(spg, ctx)
+    try:
+        ctx.set_config(long_row_ub=1024)
+        st = ctx.multiply_learned(P)
+        Cl = ctx.get_c()
+    finally:
+        P.close()
+    print(f"  nnz {Cp[0][-1]}, learned stats {st}")
+    assert st["unrecovered"] == 0
+    same_c(Cl, Cp)
+
+
+def test_stencil_64_full(spg, gpu_ctx):
+    """27-point stencil on 64^3: closed forms and linearity, both paths."""
+    print("case full stencil 64^3", flush=True)
+    ctx = gpu_ctx
+    S = spg.gen_stencil27(64, 64, 64, 3)
+    ctx.set_config(tile_log2=13, item_products=8192, long_row_ub=4096, xcd_swizzle=1, timing=1,
+                   expand_mode=spg.EXPAND_ROWS, heavy_factor=50, rank_min_count=128)
+    ctx.set_operands_from_coo(S, S)
+    nprod = ctx.init_gemm()
+    assert nprod == 566 ** 3
+    ctx.multiply()
+    Cp = ctx.get_c()
+    assert Cp[0][-1] == 314 ** 3
+    check_structure(Cp, ctx.upper_bound(S.num_rows), S.num_cols, 566 ** 3)
+    Sh = csr_host(S, (S.num_rows, S.num_cols))
+    check_linear(Cp, Sh, Sh, S.num_cols, 2)
+    P = predictor(spg, ctx)
+    try:
+        st = ctx.multiply_learned(P)
+        Cl = ctx.get_c()
+    finally:
+        P.close()
+    assert st["unrecovered"] == 0
+    same_c(Cl, Cp)
+
+
+def test_rect_rmat_full(spg, gpu_ctx):
+    """A = 2^20 x 2^18 R-MAT (8 * 2^20 draws), B = R-MAT scale 18, ef 8: the
+    learned path (the workload's headline) checked by linearity and against
+    the plain path."""
+    print("case full rectangular R-MAT", flush=True)
+    ctx = gpu_ctx
+    A = spg.gen_rmat(20, 18, 8 << 20, seed=4)
+    B = spg.gen_rmat(18, 18, 8 << 18, seed=5)
+    ctx.set_config(tile_log2=13, item_products=8192, long_row_ub=1024, xcd_swizzle=1, timing=1,
+                   expand_mode=spg.EXPAND_KBINS, heavy_factor=50, rank_min_count=128)
+    ctx.set_operands_from_coo(A, B)
+    P = predictor(spg, ctx)
+    try:
+        st = ctx.multiply_learned(P)
+        Cl = ctx.get_c()
+    finally:
+        P.close()
+    assert st["unrecovered"] == 0
+    check_structure(Cl, ctx.upper_bound(A.num_rows), B.num_cols, nprod_of(A, B))
+    check_linear(Cl, csr_host(A, (A.num_rows, A.num_cols)), csr_host(B, (B.num_rows, B.num_cols)),
+                 B.num_cols, 3)
+    ctx.multiply()
+    same_c(ctx.get_c(), Cl)
+
+
+def test_batch256_full(spg, gpu_ctx):
+    """The 256-product batch: learned + batched (graph) against plain
+    batched, every product checked by structure and linearity."""
+    print("case full batch256", flush=True)
+    import bench
+    ctx = gpu_ctx
+    pairs = bench.make_batch(256, 5)
+    ctx.set_config(tile_log2=13, item_products=8192, long_row_ub=1024, xcd_swizzle=1, timing=1,
+                   expand_mode=spg.EXPAND_KBINS, heavy_factor=50, rank_min_count=128)
+    ctx.batch_set_operands(pairs)
+    ctx.batch_multiply()
+    plain = [ctx.batch_get_c(b) for b in range(len(pairs))]
+    P = predictor(spg, ctx)
+    try:
+        st = ctx.batch_multiply(predictor=P, graph=True)
+        ctx.synchronize()
+    finally:
+        P.close()
+    assert st["unrecovered"] == 0
+    for b, (A, B) in enumerate(pairs):
+        Cl = ctx.batch_get_c(b)
+        same_c(Cl, plain[b])
+        # upper bound of every row with duplicates kept, as the library counts
+        blen = np.bincount(B.rows, minlength=B.num_rows)
+        ub = np.bincount(A.rows, weights=blen[A.cols], minlength=A.num_rows).astype(np.int64)
+        check_structure(Cl, ub, B.num_cols, nprod_of(A, B))
+        check_linear(Cl, csr_host(A, (A.num_rows, A.num_cols)),
+                     csr_host(B, (B.num_rows, B.num_cols)), B.num_cols, 10 + b)
