@@ -2789,11 +2789,21 @@ int launch_merge(spg_context* c) {
 // then give C's row offsets and the rows are compacted into CSR.
 
 // Adds the number of lanes with flag set to *ctr (one atomic per wave).
-__device__ __forceinline__ void wave_count_add(unsigned long long* ctr, bool flag) {
-    const unsigned long long mask = __ballot(flag);
-    const unsigned long long act = __ballot(1);
-    if (mask && lane_id() == __ffsll((long long)act) - 1)
-        atomicAdd(ctr, (unsigned long long)__popcll(mask));
+// Adds every thread's count to *ctr with one atomic per workgroup (kBlock
+// threads, all of which call it): the atomics of a whole grid on one
+// address serialize (one per wave cost 0.9 ms for 60K waves).
+__device__ __forceinline__ void block_count_add(unsigned long long* ctr, unsigned n) {
+    __shared__ unsigned long long part[kBlock / 64];
+    unsigned long long x = n;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (lane_id() == 0) part[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < kBlock / 64; ++w) t += part[w];
+        if (t) atomicAdd(ctr, t);
+    }
 }
 
 __device__ __forceinline__ int hash_class_for(long long cap) {
@@ -2840,14 +2850,14 @@ __global__ void k_learned_class(long long V, const int* __restrict__ vrow_row,
                                 const float* __restrict__ pred, int W, int rank_min, float margin,
                                 int slack, int tight, long long* __restrict__ vcap,
                                 int* __restrict__ cls, unsigned long long* __restrict__ nlive) {
-    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    bool live = false;
-    if (v < V) {
+    unsigned nl = 0;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+         v += (long long)gridDim.x * blockDim.x) {
         const long long u = vub[v];
         long long cap = 0;
         int k = -1;
         if (u > 0) {
-            live = true;
+            ++nl;
             const int width = vwid[v];
             const long long bound = min(u, (long long)width);
             const int i = vrow_row[v];
@@ -2887,7 +2897,7 @@ __global__ void k_learned_class(long long V, const int* __restrict__ vrow_row,
         vcap[v] = cap;
         cls[v] = k;
     }
-    wave_count_add(nlive, live);
+    block_count_add(nlive, nl);
 }
 
 // Rows that did not fit their capacity (count > capacity, or -1 from a full
@@ -2898,11 +2908,12 @@ __global__ void k_learned_check(long long V, const long long* __restrict__ vcnt,
                                 const long long* __restrict__ vub, const int* __restrict__ vwid,
                                 int W, long long* __restrict__ fcap, int* __restrict__ cls,
                                 unsigned long long* __restrict__ nover) {
-    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    bool over = false;
-    if (v < V) {
+    unsigned no = 0;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+         v += (long long)gridDim.x * blockDim.x) {
         const long long cap = pbase[v + 1] - pbase[v], cnt = vcnt[v];
-        over = cap > 0 && (cnt < 0 || cnt > cap);
+        const bool over = cap > 0 && (cnt < 0 || cnt > cap);
+        no += over ? 1u : 0u;
         long long f = 0;
         int k = -1;
         if (over) {
@@ -2918,7 +2929,7 @@ __global__ void k_learned_check(long long V, const long long* __restrict__ vcnt,
         fcap[v] = f;
         cls[v] = k;
     }
-    wave_count_add(nover, over);
+    block_count_add(nover, no);
 }
 
 // Moves every virtual row's columns and values from the padded (or the
@@ -3068,7 +3079,8 @@ int launch_learned_merge(spg_context* c, spg_learned_stats* st) {
     long long* pbase = (long long*)c->pbase.p;
     long long* fbase = (long long*)c->fbase.p;
     long long* vcnt = (long long*)c->vcnt.p;
-    const unsigned gV = (unsigned)((V + kBlock - 1) / kBlock);
+    // grid-stride: few workgroups, so few atomics on the row counters
+    const unsigned gV = persistent_grid(c, (V + kBlock - 1) / kBlock, 4);
 
     tmark(c, SPG_PHASE_MERGE, 0);
     SPG_TRY(fill<unsigned long long>(c, ctr, 0ull, 64));
