@@ -704,7 +704,16 @@ __global__ __launch_bounds__(kBlock) void k_tr_bucket(
     const int p0 = O[(long long)b * ntiles], p1 = O[(long long)(b + 1) * ntiles];
     for (int i = threadIdx.x; i < nrow; i += kBlock) rc[i] = 0;
     __syncthreads();
-    for (int p = p0 + threadIdx.x; p < p1; p += kBlock) atomicAdd(&rc[tr[p] - r0], 1);
+    // four entries per thread and pass, their loads issued together (a
+    // position past the bucket reads its last entry and is masked)
+    for (int p = p0 + threadIdx.x; p < p1; p += 4 * kBlock) {
+        int r[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) r[u] = tr[min(p + u * kBlock, p1 - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (p + u * kBlock < p1) atomicAdd(&rc[r[u] - r0], 1);
+    }
     __syncthreads();
     // exclusive scan of the bucket's row counts: consecutive counters per thread
     const int per = (nrow + kBlock - 1) / kBlock;
@@ -721,10 +730,24 @@ __global__ __launch_bounds__(kBlock) void k_tr_bucket(
     }
     if (b == NB - 1 && threadIdx.x == 0) rptr[m] = p1;
     __syncthreads();
-    for (int p = p0 + threadIdx.x; p < p1; p += kBlock) {
-        const int pos = atomicAdd(&rc[tr[p] - r0], 1);
-        rk[p0 + pos] = tk[p];
-        rv[p0 + pos] = tv[p];
+    for (int p = p0 + threadIdx.x; p < p1; p += 4 * kBlock) {
+        int r[4], k[4];
+        float x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = min(p + u * kBlock, p1 - 1);
+            r[u] = tr[q];
+            k[u] = tk[q];
+            x[u] = tv[q];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (p + u * kBlock < p1) {
+                const int pos = atomicAdd(&rc[r[u] - r0], 1);
+                rk[p0 + pos] = k[u];
+                rv[p0 + pos] = x[u];
+            }
+        }
     }
 }
 
