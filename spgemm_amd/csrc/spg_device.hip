@@ -459,12 +459,15 @@ __global__ __launch_bounds__(kBlock) void k_row_bound(const int* __restrict__ rp
         long long s = 0, s2 = 0;
         int mx = 0;
         if (!lng) {
-            for (int e = e0 + sub; e < e1; e += kGroup) {
-                const int k = rk[e];
-                const long long b = b_ptr[k + 1] - b_ptr[k];
-                s += b;
-                s2 += b * b;
-                mx = max(mx, (int)b);
+            // two entries per lane and pass, their loads issued together (the
+            // second, past the row, reads the row's last entry and is masked)
+            for (int e = e0 + sub; e < e1; e += 2 * kGroup) {
+                const int ka = rk[e], kb = rk[min(e + kGroup, e1 - 1)];
+                const int a0 = b_ptr[ka], a1 = b_ptr[ka + 1], c0 = b_ptr[kb], c1 = b_ptr[kb + 1];
+                const long long ba = a1 - a0, bb = e + kGroup < e1 ? c1 - c0 : 0;
+                s += ba + bb;
+                s2 += ba * ba + bb * bb;
+                mx = max(mx, (int)max(ba, bb));
             }
         }
 #pragma unroll
@@ -483,12 +486,22 @@ __global__ __launch_bounds__(kBlock) void k_row_bound(const int* __restrict__ rp
         const int b0 = __shfl(e0, l, 64), b1 = __shfl(e1, l, 64);
         long long s = 0, s2 = 0;
         int mx = 0;
-        for (int e = b0 + lane; e < b1; e += 64) {
-            const int k = rk[e];
-            const long long b = b_ptr[k + 1] - b_ptr[k];
-            s += b;
-            s2 += b * b;
-            mx = max(mx, (int)b);
+        for (int e = b0 + lane; e < b1; e += 4 * 64) {  // four entries per lane and pass
+            int k[4], p0[4], p1[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) k[u] = rk[min(e + u * 64, b1 - 1)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                p0[u] = b_ptr[k[u]];
+                p1[u] = b_ptr[k[u] + 1];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const long long b = e + u * 64 < b1 ? p1[u] - p0[u] : 0;
+                s += b;
+                s2 += b * b;
+                mx = max(mx, (int)b);
+            }
         }
         s = wave_sum(s);
         s2 = wave_sum(s2);
