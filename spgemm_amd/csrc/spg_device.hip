@@ -2988,6 +2988,7 @@ __global__ __launch_bounds__(kPT) void k_learned_compact(
         const long long v = grp * 64 + lane;
         long long cnt = 0, src = 0, o = 0;
         int from_fb = 0;
+        bool clamped = false;
         if (v < V) {
             cnt = vcnt[v];
             o = vout[v];
@@ -3004,6 +3005,7 @@ __global__ __launch_bounds__(kPT) void k_learned_compact(
             if (cnt > lim) {  // cannot happen: the fallback capacity is an upper bound
                 atomicAdd(unrecovered, 1ull);
                 cnt = lim;
+                clamped = true;
             }
             if (cnt < 0) cnt = 0;
         } else if (V > 0) {
@@ -3014,6 +3016,7 @@ __global__ __launch_bounds__(kPT) void k_learned_compact(
         const long long o0 = __shfl(o, 0, 64);
         const int rel = (int)(o - o0);
         const int rend = rel + (int)cnt;  // end of the row's copied range
+        const bool clean = __ballot(clamped) == 0;  // no row of the group was cut short
         const int total = __shfl((int)(o - o0 + cnt), 63, 64);
         if (total >= 64 * 256) {
             // long rows: one row after the other, the whole wave per row
@@ -3024,12 +3027,29 @@ __global__ __launch_bounds__(kPT) void k_learned_compact(
                 const bool fq = __shfl(from_fb, qq, 64) != 0;
                 const int* __restrict__ sc = fq ? fcol + sq : pcol + sq;
                 const float* __restrict__ sv = fq ? fval + sq : pval + sq;
-                for (int e0 = 0; e0 < cq; e0 += 4 * 64) {
+                // whole passes without lane conditions (a store under one would
+                // make every later store wait for the earlier ones), then the tail
+                const int full = cq & ~(4 * 64 - 1);
+                for (int e0 = 0; e0 < full; e0 += 4 * 64) {
                     int c4[4];
                     float x4[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
-                        const int e = e0 + u * 64 + lane;
+                        c4[u] = sc[e0 + u * 64 + lane];
+                        x4[u] = sv[e0 + u * 64 + lane];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        ccol[oq + e0 + u * 64 + lane] = c4[u];
+                        cval[oq + e0 + u * 64 + lane] = x4[u];
+                    }
+                }
+                if (full < cq) {
+                    int c4[4];
+                    float x4[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int e = full + u * 64 + lane;
                         if (e < cq) {
                             c4[u] = sc[e];
                             x4[u] = sv[e];
@@ -3037,7 +3057,7 @@ __global__ __launch_bounds__(kPT) void k_learned_compact(
                     }
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
-                        const int e = e0 + u * 64 + lane;
+                        const int e = full + u * 64 + lane;
                         if (e < cq) {
                             ccol[oq + e] = c4[u];
                             cval[oq + e] = x4[u];
@@ -3065,19 +3085,40 @@ __global__ __launch_bounds__(kPT) void k_learned_compact(
             int cc[4];
             float xx[4];
             bool ok[4];
+            long long sp[4];
+            bool fp[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 // every shuffle runs with all lanes active: a lane that is
                 // inactive in a bpermute does not supply its value
-                const long long s = __shfl(src, q[u], 64) + (p[u] - __shfl(rel, q[u], 64));
-                const bool f = __shfl(from_fb, q[u], 64) != 0;
+                sp[u] = __shfl(src, q[u], 64) + (p[u] - __shfl(rel, q[u], 64));
+                fp[u] = __shfl(from_fb, q[u], 64) != 0;
                 const int end = __shfl(rend, q[u], 64);
                 ok[u] = p[u] < total && p[u] < end;
+            }
+            if (clean && pb + 4 * 64 <= total) {
+                // a whole pass of a group without gaps: every position is
+                // copied, with no lane condition (a store under one would make
+                // every later store wait for the earlier ones)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    cc[u] = (fp[u] ? fcol : pcol)[sp[u]];
+                    xx[u] = (fp[u] ? fval : pval)[sp[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    ccol[o0 + p[u]] = cc[u];
+                    cval[o0 + p[u]] = xx[u];
+                }
+                continue;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
                 cc[u] = 0;
                 xx[u] = 0.f;
                 if (ok[u]) {
-                    cc[u] = f ? fcol[s] : pcol[s];
-                    xx[u] = f ? fval[s] : pval[s];
+                    cc[u] = fp[u] ? fcol[sp[u]] : pcol[sp[u]];
+                    xx[u] = fp[u] ? fval[sp[u]] : pval[sp[u]];
                 }
             }
 #pragma unroll
