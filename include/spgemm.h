@@ -190,6 +190,11 @@ typedef struct spg_config {
                                   src/mm.cu:138-148 counts rows) or
                                   SPG_TRANSPOSE_AUTO (the sort from 2^20 entries
                                   on, the atomic one below: fewer launches)     */
+    int32_t rowchunk_ub;       /* SPG_EXPAND_KBINS: tiled rows whose upper bound
+                                  is below this (0: none) are expanded row by
+                                  row, in chunks of entries, instead of by k
+                                  (as SPG_EXPAND_ROWS expands every tiled row):
+                                  their tiles are filled front to back           */
 } spg_config;
 
 #define SPG_TRANSPOSE_AUTO 0

@@ -74,7 +74,7 @@ class Config(Structure):
                 ("xcd_swizzle", c_int32), ("timing", c_int32), ("expand_mode", c_int32),
                 ("pred_margin", c_float), ("pred_slack", c_int32), ("pred_tight", c_int32),
                 ("heavy_factor", c_int32), ("rank_min_count", c_int32),
-                ("transpose_mode", c_int32)]
+                ("transpose_mode", c_int32), ("rowchunk_ub", c_int32)]
 
 
 class LearnedStats(Structure):

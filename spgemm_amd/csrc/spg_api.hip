@@ -255,6 +255,7 @@ int spgSetConfig(spg_context* c, const spg_config* cfg) {
         return fail(c, SPG_EINVAL, "rank_min_count must be in [0, 4096]");
     if (cfg->transpose_mode < SPG_TRANSPOSE_AUTO || cfg->transpose_mode > SPG_TRANSPOSE_SORT)
         return fail(c, SPG_EINVAL, "transpose_mode must be AUTO, ATOMIC or SORT");
+    if (cfg->rowchunk_ub < 0) return fail(c, SPG_EINVAL, "rowchunk_ub must be >= 0");
     c->cfg = *cfg;
     c->stage = std::min(c->stage, 0);
     graph_release(c);

@@ -1027,6 +1027,7 @@ struct ExpandArgs {
     int swizzle;
     int* icol;
     float* ival;
+    const int* rcskip;  // rows expanded in chunks (key >= 0) are skipped, or null
 };
 
 struct ExpandLds {
@@ -1035,6 +1036,7 @@ struct ExpandLds {
     int sflag[kBlock];
     long long tbase[kTBCap];
     int sbs[kTBCap / 2 + 1];
+    int sh[kBlock / 64];
 };
 
 // Expands A entries [e_begin, e_begin+ne) of column k against B row k's
@@ -1073,20 +1075,32 @@ __device__ void expand_work(const ExpandArgs& g, ExpandLds& L, int k, long long 
         }
     }
     for (long long eb = 0; eb < ne; eb += R) {
-        const int cnt = (int)min((long long)R, ne - eb);
+        int cnt = (int)min((long long)R, ne - eb);
         if (eb > 0) __syncthreads();
-        if (threadIdx.x < cnt) {
+        // the round's entries (those of rows expanded in chunks left out,
+        // the others packed to the front), each with its slot reservation
+        int i = 0, q = threadIdx.x;
+        float av = 0.f;
+        bool keep = threadIdx.x < cnt;
+        if (keep) {
             const long long e = e_begin + eb + threadIdx.x;
-            const int i = g.a_idx[e];
-            L.sa[threadIdx.x] = g.a_val[e];
+            i = g.a_idx[e];
+            av = g.a_val[e];
+        }
+        if (g.rcskip) {  // uniform
+            if (keep) keep = g.rcskip[i] < 0;
+            q = block_exscan<int>(keep ? 1 : 0, &cnt, L.sh);
+        }
+        if (keep) {
+            L.sa[q] = av;
             const int vs = g.vstart[i];
             if (g.vstart[i + 1] - vs == 1) {
                 const int off = atomicAdd(&g.cursor[vs], nj);
-                L.sbase[threadIdx.x] = g.vbase[vs] + off;
-                L.sflag[threadIdx.x] = 0;
+                L.sbase[q] = g.vbase[vs] + off;
+                L.sflag[q] = 0;
             } else {
-                L.sbase[threadIdx.x] = vs;
-                L.sflag[threadIdx.x] = 1;
+                L.sbase[q] = vs;
+                L.sflag[q] = 1;
             }
         }
         __syncthreads();
@@ -1260,21 +1274,6 @@ static ExpandArgs expand_args(spg_context* c) {
     g.icol = inter_cols(c);
     g.ival = inter_vals(c);
     return g;
-}
-
-int launch_expand_binned(spg_context* c) {
-    ExpandArgs g = expand_args(c);
-    g.np = c->kbin_off_h[SPG_NUM_KBINS];
-    g.nitems = c->kitems_bin_h[SPG_NUM_KBINS];
-    if (g.nitems <= 0) return SPG_OK;
-    unsigned long long* ctr = (unsigned long long*)c->counters.p + 8;
-    SPG_TRY(fill<unsigned long long>(c, ctr, 0ull, 8));
-    const long long nchunks = (g.nitems + kItemChunk - 1) / kItemChunk;
-    unsigned grid = persistent_grid(c, nchunks, kExpandWaves);
-    if (g.swizzle) grid = (grid + 7) & ~7u;  // every XCD part gets workgroups
-    hipLaunchKernelGGL(k_expand, dim3(grid), dim3(kBlock), 0, c->stream, g, ctr);
-    SPG_HIP(hipGetLastError());
-    return SPG_OK;
 }
 
 int launch_expand_simple(spg_context* c) {
@@ -1493,19 +1492,318 @@ __global__ __launch_bounds__(kBlock) void k_expand_rows_wg(RowsArgs g, const int
     }
 }
 
+// Tiled rows with at most kRCMaxTiles tiles, in chunks of E consecutive
+// entries (E = min(64, kRCTab / tiles)); a workgroup takes one chunk at a
+// time from a queue (the heaviest rows' chunks first; the next chunk is
+// fetched while the current one is expanded).  For every chunk and tile,
+// the pieces of the chunk's entries inside the tile (ranges of B's split
+// points) are laid end to end into one run, reserved with one atomic on the
+// tile's cursor, and each wave writes a contiguous share of the chunk's
+// (tile, 64-slot block) pairs: the 64 lanes of one store cover 64
+// consecutive slots of one run, each lane finding its entry by a binary
+// search over the run offsets and gathering its B column and value.  The
+// runs of a row's chunks are reserved at nearly the same time, so its lines
+// are completed within microseconds, where the binned expansion's short
+// pieces leave partial lines written far apart in time (L2 write-back for
+// the 23.5 GB of intermediates of R-MAT scale 18: 24.3 GB here, 29.0 GB
+// binned); the price is that B is read once per product (from L2 or the
+// last-level cache) instead of once per work item, so the two mappings end
+// up equally fast there (9.6 ms).  Lanes past a run's end repeat its
+// last product (same slot, same data), so no store is under a lane
+// condition.
+constexpr int kRCTab = 2048;      // (entry, tile) slot bases of one chunk
+constexpr int kRCMaxTiles = 256;  // rows with more tiles: per-vrow kernels
+constexpr int kRCMaxE = 64;       // entries per chunk
+
+__device__ __forceinline__ int rc_chunk_entries(int Ti) { return min(kRCMaxE, kRCTab / Ti); }
+
+struct RCArgs {
+    const int* rk;
+    const float* rv;
+    const int* b_idx;
+    const float* b_val;
+    const int* bsplit;
+    const long long* vbase;
+    int* cursor;
+    int T;
+    int* icol;
+    float* ival;
+};
+
+// Chunks of every listed row.
+__global__ void k_rc_chunks(const int* __restrict__ rows, long long nrows,
+                            const int* __restrict__ rptr, const int* __restrict__ vstart,
+                            int* __restrict__ nch) {
+    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const int i = rows[r];
+    const int E = rc_chunk_entries(vstart[i + 1] - vstart[i]);
+    nch[r] = (rptr[i + 1] - rptr[i] + E - 1) / E;
+}
+
+// Every chunk's (first vrow, tiles, first entry, entries).
+__global__ void k_rc_units(const int* __restrict__ rows, long long nrows,
+                           const int* __restrict__ rptr, const int* __restrict__ vstart,
+                           const int* __restrict__ choff, int4* __restrict__ units) {
+    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const int i = rows[r];
+    const int vs = vstart[i], Ti = vstart[i + 1] - vs;
+    const int E = rc_chunk_entries(Ti);
+    const int e0 = rptr[i], e1 = rptr[i + 1];
+    int u = choff[r];
+    for (int c0 = e0; c0 < e1; c0 += E) units[u++] = make_int4(vs, Ti, c0, min(E, e1 - c0));
+}
+
+__global__ __launch_bounds__(kBlock) void k_expand_rowchunks(RCArgs g, const int4* __restrict__ units,
+                                                              int nunits,
+                                                              unsigned long long* __restrict__ ctr) {
+    __shared__ int pref[kRCTab + kRCMaxTiles];  // [e][t]: tile t's run offset of entry e (e <= ne)
+    __shared__ int xs[kRCTab];                  // [e][t]: B position of the piece's start
+    __shared__ long long sbase[kRCMaxTiles];
+    __shared__ int gsum[kBlock];
+    __shared__ int tpb[kRCMaxTiles + 1];        // 64-slot blocks of the tiles, scanned
+    __shared__ int sk[kRCMaxE];
+    __shared__ float sa[kRCMaxE];
+    __shared__ int sh[kBlock / 64];
+    __shared__ int snext;
+    constexpr int NW = kBlock / 64;
+    const int wv = threadIdx.x >> 6, lane = lane_id();
+    const int* __restrict__ bidx = g.b_idx;
+    const float* __restrict__ bval = g.b_val;
+    int* __restrict__ icol = g.icol;
+    float* __restrict__ ival = g.ival;
+    // the next unit is taken from the queue (and its description loaded)
+    // while the current one is expanded
+    if (threadIdx.x == 0) snext = (int)atomicAdd(ctr, 1ull);
+    __syncthreads();
+    int u = snext;
+    int4 inf = u < nunits ? units[u] : make_int4(0, 1, 0, 0);
+    while (u < nunits) {
+        unsigned long long nx = 0;
+        if (threadIdx.x == 0) nx = atomicAdd(ctr, 1ull);
+        const int vs = inf.x, Ti = inf.y, c0 = inf.z, ne = inf.w;  // Ti in 2 .. kRCMaxTiles
+        const int E = rc_chunk_entries(Ti);
+        const int NG = kBlock / Ti;         // entry groups of the prefix pass
+        const int EPG = (E + NG - 1) / NG;  // entries per group
+        const int t = threadIdx.x % Ti, eg = threadIdx.x / Ti;
+        if (threadIdx.x < ne) {
+            sk[threadIdx.x] = g.rk[c0 + threadIdx.x];
+            sa[threadIdx.x] = g.rv[c0 + threadIdx.x];
+        }
+        if (threadIdx.x == 0) snext = (int)nx;
+        __syncthreads();
+        const int un = snext;
+        const int4 infn = un < nunits ? units[un] : make_int4(0, 1, 0, 0);
+        const int ea = eg * EPG, eb = min(ea + EPG, ne);
+        if (eg < NG) {  // tile t's piece of every entry in [ea, eb): start and run offset
+            int run = 0;
+#pragma unroll 4
+            for (int e = ea; e < eb; ++e) {
+                const int* bs = g.bsplit + (long long)sk[e] * (g.T + 1) + t;
+                const int x = bs[0], y = bs[1];
+                xs[e * Ti + t] = x;
+                pref[e * Ti + t] = run;
+                run += y - x;
+            }
+            gsum[eg * Ti + t] = run;
+        }
+        __syncthreads();
+        int nbt = 0;
+        if (threadIdx.x < Ti) {  // the chunk's run of every tile, reserved
+            int sum = 0;
+            for (int q = 0; q < NG; ++q) sum += gsum[q * Ti + threadIdx.x];
+            long long base = 0;
+            if (sum > 0) base = g.vbase[vs + threadIdx.x] + atomicAdd(&g.cursor[vs + threadIdx.x], sum);
+            sbase[threadIdx.x] = base;
+            pref[ne * Ti + threadIdx.x] = sum;
+            nbt = (sum + 63) >> 6;
+        }
+        int nbk;
+        const int tb = block_exscan<int>(nbt, &nbk, sh);
+        if (threadIdx.x < Ti) tpb[threadIdx.x] = tb;
+        if (threadIdx.x == 0) tpb[Ti] = nbk;
+        if (eg < NG) {
+            int pre = 0;
+            for (int q = 0; q < eg; ++q) pre += gsum[q * Ti + t];
+            for (int e = ea; e < eb; ++e) pref[e * Ti + t] += pre;
+        }
+        __syncthreads();
+        // (tile, 64-slot block) pairs, a contiguous share per wave: the lanes
+        // of one store cover 64 consecutive slots of one tile's run
+        const int per = (nbk + NW - 1) / NW;
+        const int pb = min(nbk, wv * per), pe = min(nbk, pb + per);
+        int tc = 0;
+        for (int p0 = pb; p0 < pe; p0 += 4) {
+            int tv[4], pos[4], ev[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int P = min(p0 + v, pe - 1);
+                while (tpb[tc + 1] <= P) ++tc;
+                tv[v] = tc;
+                pos[v] = min(((P - tpb[tc]) << 6) + lane, pref[ne * Ti + tc] - 1);
+                ev[v] = 0;  // largest entry whose run offset is <= pos
+            }
+#pragma unroll
+            for (int step = kRCMaxE / 2; step > 0; step >>= 1) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int cand = ev[v] + step;
+                    if (cand < ne && pref[cand * Ti + tv[v]] <= pos[v]) ev[v] = cand;
+                }
+            }
+            int col[4];
+            float bv[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int q = xs[ev[v] * Ti + tv[v]] + pos[v] - pref[ev[v] * Ti + tv[v]];
+                col[v] = bidx[q];
+                bv[v] = bval[q];
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const long long w = islot(sbase[tv[v]] + pos[v]);
+                icol[w] = col[v];
+                ival[w] = __fmul_rn(sa[ev[v]], bv[v]);
+            }
+        }
+        __syncthreads();  // the unit's LDS tables are free
+        u = un;
+        inf = infn;
+    }
+}
+
+// Row-chunk classes: tiled rows with at most kRCMaxTiles tiles, by bound
+// (>= 2^20, >= 2^16, the rest), so that the heaviest rows start first.
+__global__ void k_rc_key(int m, const int* __restrict__ vstart, const long long* __restrict__ ub,
+                         long long max_ub, int* __restrict__ key) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int Ti = vstart[i + 1] - vstart[i];
+    const long long u = ub[i];
+    key[i] = (Ti > 1 && Ti <= kRCMaxTiles && u < max_ub)
+                 ? (u >= (1LL << 20) ? 0 : (u >= (1LL << 16) ? 1 : 2))
+                 : -1;
+}
+
+// The row-chunk work of an expansion: the rows' keys (>= 0: the row is
+// expanded in chunks; the binned kernel skips its entries), the chunk table
+// and its size.
+struct RCPlan {
+    const int* rkey = nullptr;
+    const int4* units = nullptr;
+    int nunits = 0;
+};
+
+static int plan_rowchunks(spg_context* c, long long max_ub, RCPlan& pl) {
+    const int m = c->m;
+    pl = RCPlan{};
+    if (!(c->have_long && c->T > 1 && m > 0 && max_ub > 0)) return SPG_OK;
+    int* rkey = (int*)scratch_alloc(c, sizeof(int) * (size_t)m);
+    int* rlist = (int*)scratch_alloc(c, sizeof(int) * (size_t)m);
+    int64_t* roff = (int64_t*)scratch_alloc(c, sizeof(int64_t) * 4);
+    if (!rkey || !rlist || !roff) return fail(c, SPG_ENOMEM, "row-chunk scratch");
+    hipLaunchKernelGGL(k_rc_key, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, m,
+                       (const int*)c->vstart.p, (const long long*)c->ub.p, max_ub, rkey);
+    SPG_HIP(hipGetLastError());
+    pl.rkey = rkey;
+    // heaviest rows first
+    SPG_TRY(stable_partition(c, rkey, m, 3, rlist, roff));
+    long long roff_h[4];
+    SPG_TRY(readback(c, roff, sizeof(roff_h), roff_h));
+    const long long nrc = roff_h[3];
+    if (nrc <= 0) return SPG_OK;
+    int* nch = (int*)scratch_alloc(c, sizeof(int) * (size_t)nrc);
+    int* choff = (int*)scratch_alloc(c, sizeof(int) * (size_t)(nrc + 1));
+    if (!nch || !choff) return fail(c, SPG_ENOMEM, "row-chunk scratch");
+    const unsigned gr = (unsigned)((nrc + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_rc_chunks, dim3(gr), dim3(kBlock), 0, c->stream, (const int*)rlist, nrc,
+                       (const int*)c->arow_ptr.p, (const int*)c->vstart.p, nch);
+    SPG_HIP(hipGetLastError());
+    SPG_TRY(scan_i32(c, nch, choff, nrc));
+    int nunits = 0;
+    SPG_TRY(readback(c, choff + nrc, sizeof(int), &nunits));
+    if (nunits <= 0) return SPG_OK;
+    int4* units = (int4*)scratch_alloc(c, sizeof(int4) * (size_t)nunits);
+    if (!units) return fail(c, SPG_ENOMEM, "row-chunk scratch");
+    hipLaunchKernelGGL(k_rc_units, dim3(gr), dim3(kBlock), 0, c->stream, (const int*)rlist, nrc,
+                       (const int*)c->arow_ptr.p, (const int*)c->vstart.p, (const int*)choff, units);
+    SPG_HIP(hipGetLastError());
+    pl.units = units;
+    pl.nunits = nunits;
+    return fill<int>(c, (int*)c->cursor.p, 0, c->V);
+}
+
+static int launch_rowchunks(spg_context* c, const RCPlan& pl) {
+    if (pl.nunits <= 0) return SPG_OK;
+    unsigned long long* ctr = (unsigned long long*)c->counters.p + 32;
+    SPG_TRY(fill<unsigned long long>(c, ctr, 0ull, 1));
+    RCArgs h{};
+    h.rk = (const int*)c->arow_k.p;
+    h.rv = (const float*)c->arow_v.p;
+    h.b_idx = c->B.idx;
+    h.b_val = c->B.val;
+    h.bsplit = (const int*)c->bsplit.p;
+    h.vbase = (const long long*)c->vbase.p;
+    h.cursor = (int*)c->cursor.p;
+    h.T = c->T;
+    h.icol = inter_cols(c);
+    h.ival = inter_vals(c);
+    hipLaunchKernelGGL(k_expand_rowchunks, dim3(persistent_grid(c, pl.nunits, 6)), dim3(kBlock), 0,
+                       c->stream, h, pl.units, pl.nunits, ctr);
+    SPG_HIP(hipGetLastError());
+    return SPG_OK;
+}
+
+int launch_expand_binned(spg_context* c) {
+    ExpandArgs g = expand_args(c);
+    g.np = c->kbin_off_h[SPG_NUM_KBINS];
+    g.nitems = c->kitems_bin_h[SPG_NUM_KBINS];
+    if (g.nitems <= 0) return SPG_OK;
+    // tiled rows with a bound below rowchunk_ub: expanded in chunks first
+    RCPlan pl;
+    SPG_TRY(plan_rowchunks(c, c->cfg.rowchunk_ub, pl));
+    SPG_TRY(launch_rowchunks(c, pl));
+    g.rcskip = pl.nunits > 0 ? pl.rkey : nullptr;
+    unsigned long long* ctr = (unsigned long long*)c->counters.p + 8;
+    SPG_TRY(fill<unsigned long long>(c, ctr, 0ull, 8));
+    const long long nchunks = (g.nitems + kItemChunk - 1) / kItemChunk;
+    unsigned grid = persistent_grid(c, nchunks, kExpandWaves);
+    if (g.swizzle) grid = (grid + 7) & ~7u;  // every XCD part gets workgroups
+    hipLaunchKernelGGL(k_expand, dim3(grid), dim3(kBlock), 0, c->stream, g, ctr);
+    SPG_HIP(hipGetLastError());
+    return SPG_OK;
+}
+
+// Per-vrow kernels: 0 wave, 1 workgroup; vrows of row-chunk rows and empty
+// vrows are dropped.
 __global__ void k_vclass_expand(long long V, const long long* __restrict__ vub,
-                                int* __restrict__ cls) {
+                                const int* __restrict__ vrow_row, const int* __restrict__ vstart,
+                                int rc, int* __restrict__ cls) {
     long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     const long long u = vub[v];
+    if (rc) {
+        const int i = vrow_row[v];
+        const int Ti = vstart[i + 1] - vstart[i];
+        if (Ti > 1 && Ti <= kRCMaxTiles) {
+            cls[v] = -1;
+            return;
+        }
+    }
     cls[v] = u == 0 ? -1 : (u <= kRowsWaveMax ? 0 : 1);
 }
 
 int launch_expand_rows(spg_context* c) {
     const long long V = c->V;
     if (V <= 0) return SPG_OK;
+    // every tiled row of at most kRCMaxTiles tiles is expanded in chunks
+    RCPlan pl;
+    SPG_TRY(plan_rowchunks(c, LLONG_MAX, pl));
+    const int rc = pl.rkey != nullptr;
     hipLaunchKernelGGL(k_vclass_expand, dim3((unsigned)((V + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                       c->stream, V, (const long long*)c->vub.p, (int*)c->vclass.p);
+                       c->stream, V, (const long long*)c->vub.p, (const int*)c->vrow_row.p,
+                       (const int*)c->vstart.p, rc, (int*)c->vclass.p);
     SPG_HIP(hipGetLastError());
     SPG_TRY(stable_partition(c, (const int32_t*)c->vclass.p, V, 2, (int32_t*)c->vlist.p,
                              (int64_t*)c->vclass_off.p));
@@ -1528,6 +1826,7 @@ int launch_expand_rows(spg_context* c) {
     g.ival = inter_vals(c);
     const int* vl = (const int*)c->vlist.p;
     tmark(c, SPG_PHASE_EXPAND, 0);
+    SPG_TRY(launch_rowchunks(c, pl));
     if (n1 > 0) {  // largest rows first
         hipLaunchKernelGGL(k_expand_rows_wg, dim3(persistent_grid(c, n1, 8)), dim3(kBlock), 0,
                            c->stream, g, vl + off_h[1], n1);

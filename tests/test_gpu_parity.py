@@ -163,6 +163,22 @@ def test_long_row_tiling(spg, orc, gpu_ctx, tile_log2, long_ub, mode):
     compare(out, oracle_all(orc, A, sh, A, sh))
 
 
+@pytest.mark.parametrize("rc_ub", [1 << 12, 1 << 30], ids=["rc_light", "rc_all"])
+@pytest.mark.parametrize("tile_log2,long_ub,s", [(6, 16, 10), (5, 1, 13)])
+def test_rowchunk_split(spg, orc, gpu_ctx, tile_log2, long_ub, s, rc_ub):
+    """The binned expansion with the tiled rows below a bound expanded in
+    row chunks (slots reserved per chunk and tile, runs written front to
+    back) and the rest by k: same intermediates and C as the oracle, with
+    16 (entries per chunk: 64) and 256 tiles (the largest row-chunk row:
+    8 entries per chunk)."""
+    print(f"case rowchunk tile_log2={tile_log2} long_ub={long_ub} s={s} rc_ub={rc_ub}", flush=True)
+    A, sh = gen(spg, "rmat", s=s, ef=8, seed=9)
+    out = run_gpu(spg, gpu_ctx, A, sh, A, sh,
+                  cfg=dict(tile_log2=tile_log2, long_row_ub=long_ub, expand_mode=spg.EXPAND_KBINS,
+                           rowchunk_ub=rc_ub))
+    compare(out, oracle_all(orc, A, sh, A, sh))
+
+
 @pytest.mark.parametrize("tmode", [2, 1], ids=["sort", "atomic"])
 def test_transpose_modes(spg, orc, gpu_ctx, tmode):
     """InitGEMM's A-by-rows transpose: the LDS counting sort and the
