@@ -8,14 +8,15 @@
 //   into T column tiles of width W = 2^tile_log2; every other row is one
 //   virtual row.  vstart[m+1] maps rows to their first virtual row, vrow_row
 //   maps back, vub/vbase give each virtual row's slot range.
-//   intermediates  one buffer of 128-byte groups of 16 products: the 16
-//   columns (int32), then the 16 values (f32); product s sits at word
-//   islot(s) of the column pointer and of the value pointer (16 words
-//   later).  A run of consecutive products fills whole groups, so the
-//   expansion's scattered runs complete twice as many cache lines as with
-//   two separate arrays, and the column-only symbolic pass reads only the
-//   column halves.  Products are grouped by virtual row (so a C row's
-//   products are contiguous too)
+//   intermediates  one buffer of 512-byte groups of 64 products: the 64
+//   columns (int32, two 128-byte lines), then the 64 values (f32); product
+//   s sits at word islot(s) of the column pointer and of the value pointer
+//   (64 words later).  The column-only symbolic pass reads whole lines of
+//   columns (with 16-product groups, which put a product's column and value
+//   in one line, it fetched the values too: 6.1 vs 3.8 ms on R-MAT scale
+//   18), and one wave store of 64 consecutive products writes two whole
+//   lines.  Products are grouped by virtual row (so a C row's products are
+//   contiguous too)
 //   C  CSR  crow[m+1] int64, ccol[nnzC] int32, cval[nnzC] f32
 //
 // Value contract: a product is the FP32 product a*b (rounded once, as
@@ -1514,6 +1515,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_rows_wg(RowsArgs g, const int
 constexpr int kRCTab = 2048;      // (entry, tile) slot bases of one chunk
 constexpr int kRCMaxTiles = 256;  // rows with more tiles: per-vrow kernels
 constexpr int kRCMaxE = 64;       // entries per chunk
+constexpr int kRCU = 8;           // (tile, block) pairs in flight per wave
 
 __device__ __forceinline__ int rc_chunk_entries(int Ti) { return min(kRCMaxE, kRCTab / Ti); }
 
@@ -1634,10 +1636,10 @@ __global__ __launch_bounds__(kBlock) void k_expand_rowchunks(RCArgs g, const int
         const int per = (nbk + NW - 1) / NW;
         const int pb = min(nbk, wv * per), pe = min(nbk, pb + per);
         int tc = 0;
-        for (int p0 = pb; p0 < pe; p0 += 4) {
-            int tv[4], pos[4], ev[4];
+        for (int p0 = pb; p0 < pe; p0 += kRCU) {
+            int tv[kRCU], pos[kRCU], ev[kRCU];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
+            for (int v = 0; v < kRCU; ++v) {
                 const int P = min(p0 + v, pe - 1);
                 while (tpb[tc + 1] <= P) ++tc;
                 tv[v] = tc;
@@ -1647,21 +1649,21 @@ __global__ __launch_bounds__(kBlock) void k_expand_rowchunks(RCArgs g, const int
 #pragma unroll
             for (int step = kRCMaxE / 2; step > 0; step >>= 1) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
+                for (int v = 0; v < kRCU; ++v) {
                     const int cand = ev[v] + step;
                     if (cand < ne && pref[cand * Ti + tv[v]] <= pos[v]) ev[v] = cand;
                 }
             }
-            int col[4];
-            float bv[4];
+            int col[kRCU];
+            float bv[kRCU];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
+            for (int v = 0; v < kRCU; ++v) {
                 const int q = xs[ev[v] * Ti + tv[v]] + pos[v] - pref[ev[v] * Ti + tv[v]];
                 col[v] = bidx[q];
                 bv[v] = bval[q];
             }
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
+            for (int v = 0; v < kRCU; ++v) {
                 const long long w = islot(sbase[tv[v]] + pos[v]);
                 icol[w] = col[v];
                 ival[w] = __fmul_rn(sa[ev[v]], bv[v]);

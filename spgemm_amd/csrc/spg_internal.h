@@ -83,9 +83,9 @@ struct spg_context {
     spg::DevBuf kbin, kperm, kbin_off, kitems, kitem_off;
     int64_t kbin_off_h[SPG_NUM_KBINS + 1] = {};
     int64_t kitems_bin_h[SPG_NUM_KBINS + 1] = {};  // item offsets at bin starts
-    // intermediates, in groups of 16 products: each 128-byte group holds the
-    // 16 columns, then the 16 values (product s: word ((s >> 4) << 5) + (s & 15)
-    // of the column half, 16 words later its value)
+    // intermediates, in groups of 64 products: each 512-byte group holds the
+    // 64 columns, then the 64 values (product s: word ((s >> 6) << 7) + (s & 63)
+    // of the column half, 64 words later its value)
     spg::DevBuf inter;
     // symbolic / merge
     spg::DevBuf vclass, vlist, vclass_off, vcnt, vout, crow, ccol, cval;
@@ -126,7 +126,7 @@ struct spg_context {
 };
 
 #ifndef SPG_IGROUP_LOG2
-#define SPG_IGROUP_LOG2 4
+#define SPG_IGROUP_LOG2 6
 #endif
 namespace spg {
 constexpr int kIGroupLog2 = SPG_IGROUP_LOG2;  // products per group (log2)
