@@ -1322,8 +1322,7 @@ struct RowsArgs {
     const int* rk;
     const float* rv;
     const int* b_ptr;
-    const int* b_idx;
-    const float* b_val;
+    const int2* bpair;  // B's (column, value bits) pairs
     const int* bsplit;
     const int* vrow_row;
     const int* vstart;
@@ -1351,8 +1350,7 @@ __global__ __launch_bounds__(kPT) void k_expand_rows_wave(RowsArgs g, const int*
     const int wv = threadIdx.x >> 6, lane = lane_id();
     constexpr int NWV = kPT / 64;
     const long long stride = (long long)gridDim.x * NWV;
-    const int* __restrict__ bidx = g.b_idx;
-    const float* __restrict__ bval = g.b_val;
+    const int2* __restrict__ bpair = g.bpair;
     int* __restrict__ icol = g.icol;
     float* __restrict__ ival = g.ival;
     for (long long j = (long long)blockIdx.x * NWV + wv; j < nlist; j += stride) {
@@ -1408,8 +1406,9 @@ __global__ __launch_bounds__(kPT) void k_expand_rows_wave(RowsArgs g, const int*
                 float bv[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    col[u] = p[u] < total ? bidx[src[u]] : 0;
-                    bv[u] = p[u] < total ? bval[src[u]] : 0.f;
+                    const int2 pr = p[u] < total ? bpair[src[u]] : make_int2(0, 0);
+                    col[u] = pr.x;
+                    bv[u] = __int_as_float(pr.y);
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -1475,8 +1474,9 @@ __global__ __launch_bounds__(kBlock) void k_expand_rows_wg(RowsArgs g, const int
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int src = sps[lo[u]] + (p[u] - soff[lo[u]]);
-                    col[u] = p[u] < total ? g.b_idx[src] : 0;
-                    bv[u] = p[u] < total ? g.b_val[src] : 0.f;
+                    const int2 pr = p[u] < total ? g.bpair[src] : make_int2(0, 0);
+                    col[u] = pr.x;
+                    bv[u] = __int_as_float(pr.y);
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -1522,8 +1522,7 @@ __device__ __forceinline__ int rc_chunk_entries(int Ti) { return min(kRCMaxE, kR
 struct RCArgs {
     const int* rk;
     const float* rv;
-    const int* b_idx;
-    const float* b_val;
+    const int2* bpair;  // B's (column, value bits) pairs
     const int* bsplit;
     const long long* vbase;
     int* cursor;
@@ -1571,8 +1570,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_rowchunks(RCArgs g, const int
     __shared__ int snext;
     constexpr int NW = kBlock / 64;
     const int wv = threadIdx.x >> 6, lane = lane_id();
-    const int* __restrict__ bidx = g.b_idx;
-    const float* __restrict__ bval = g.b_val;
+    const int2* __restrict__ bpair = g.bpair;
     int* __restrict__ icol = g.icol;
     float* __restrict__ ival = g.ival;
     // the next unit is taken from the queue (and its description loaded)
@@ -1659,8 +1657,9 @@ __global__ __launch_bounds__(kBlock) void k_expand_rowchunks(RCArgs g, const int
 #pragma unroll
             for (int v = 0; v < kRCU; ++v) {
                 const int q = xs[ev[v] * Ti + tv[v]] + pos[v] - pref[ev[v] * Ti + tv[v]];
-                col[v] = bidx[q];
-                bv[v] = bval[q];
+                const int2 pr = bpair[q];
+                col[v] = pr.x;
+                bv[v] = __int_as_float(pr.y);
             }
 #pragma unroll
             for (int v = 0; v < kRCU; ++v) {
@@ -1673,6 +1672,12 @@ __global__ __launch_bounds__(kBlock) void k_expand_rowchunks(RCArgs g, const int
         u = un;
         inf = infn;
     }
+}
+
+__global__ void k_interleave(const int* __restrict__ idx, const float* __restrict__ val, long long n,
+                             int2* __restrict__ out) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) out[q] = make_int2(idx[q], __float_as_int(val[q]));
 }
 
 // Row-chunk classes: tiled rows with at most kRCMaxTiles tiles, by bound
@@ -1736,6 +1741,18 @@ static int plan_rowchunks(spg_context* c, long long max_ub, RCPlan& pl) {
     return fill<int>(c, (int*)c->cursor.p, 0, c->V);
 }
 
+// B's (column, value) pairs interleaved into c->bpair.
+static int make_bpair(spg_context* c) {
+    const long long nb = c->B.nnz;
+    SPG_TRY(ensure(c, c->bpair, sizeof(int2) * (size_t)std::max<long long>(nb, 1)));
+    if (nb > 0) {
+        hipLaunchKernelGGL(k_interleave, dim3((unsigned)((nb + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           c->stream, c->B.idx, c->B.val, nb, (int2*)c->bpair.p);
+        SPG_HIP(hipGetLastError());
+    }
+    return SPG_OK;
+}
+
 static int launch_rowchunks(spg_context* c, const RCPlan& pl) {
     if (pl.nunits <= 0) return SPG_OK;
     unsigned long long* ctr = (unsigned long long*)c->counters.p + 32;
@@ -1743,8 +1760,7 @@ static int launch_rowchunks(spg_context* c, const RCPlan& pl) {
     RCArgs h{};
     h.rk = (const int*)c->arow_k.p;
     h.rv = (const float*)c->arow_v.p;
-    h.b_idx = c->B.idx;
-    h.b_val = c->B.val;
+    h.bpair = (const int2*)c->bpair.p;
     h.bsplit = (const int*)c->bsplit.p;
     h.vbase = (const long long*)c->vbase.p;
     h.cursor = (int*)c->cursor.p;
@@ -1765,6 +1781,7 @@ int launch_expand_binned(spg_context* c) {
     // tiled rows with a bound below rowchunk_ub: expanded in chunks first
     RCPlan pl;
     SPG_TRY(plan_rowchunks(c, c->cfg.rowchunk_ub, pl));
+    if (pl.nunits > 0) SPG_TRY(make_bpair(c));
     SPG_TRY(launch_rowchunks(c, pl));
     g.rcskip = pl.nunits > 0 ? pl.rkey : nullptr;
     unsigned long long* ctr = (unsigned long long*)c->counters.p + 8;
@@ -1803,6 +1820,7 @@ int launch_expand_rows(spg_context* c) {
     RCPlan pl;
     SPG_TRY(plan_rowchunks(c, LLONG_MAX, pl));
     const int rc = pl.rkey != nullptr;
+    SPG_TRY(make_bpair(c));
     hipLaunchKernelGGL(k_vclass_expand, dim3((unsigned)((V + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                        c->stream, V, (const long long*)c->vub.p, (const int*)c->vrow_row.p,
                        (const int*)c->vstart.p, rc, (int*)c->vclass.p);
@@ -1817,8 +1835,7 @@ int launch_expand_rows(spg_context* c) {
     g.rk = (const int*)c->arow_k.p;
     g.rv = (const float*)c->arow_v.p;
     g.b_ptr = c->B.ptr;
-    g.b_idx = c->B.idx;
-    g.b_val = c->B.val;
+    g.bpair = (const int2*)c->bpair.p;
     g.bsplit = (const int*)c->bsplit.p;
     g.vrow_row = (const int*)c->vrow_row.p;
     g.vstart = (const int*)c->vstart.p;

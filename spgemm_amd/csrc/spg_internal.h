@@ -87,6 +87,9 @@ struct spg_context {
     // 64 columns, then the 64 values (product s: word ((s >> 6) << 7) + (s & 63)
     // of the column half, 64 words later its value)
     spg::DevBuf inter;
+    // B's (column, value) pairs interleaved, for the row-chunk expansion's
+    // gathers (one 8-byte load per product)
+    spg::DevBuf bpair;
     // symbolic / merge
     spg::DevBuf vclass, vlist, vclass_off, vcnt, vout, crow, ccol, cval;
     // learned merge: row predictions, per-virtual-row capacities, padded
