@@ -339,12 +339,13 @@ def bench_batch(args):
 
 def bench_batch_only(args, pairs):
     """The batched runs alone (for profiling): eager plain and learned
-    passes, then graph replays of the learned pass."""
+    passes, then graph replays of the learned pass, with the learned pass's
+    settings (--tune-table entry, else tuned here first)."""
     from spgemm_amd import train_predictor as tp
     ctx = spg.Context(0)
-    ctx.set_config(expand_mode=spg.EXPAND_KBINS)
     P, _ = tp.load_or_train(ctx)
     ctx.batch_set_operands(pairs)
+    apply_tuning(ctx, args, P, True)
     for _ in range(args.warmup + args.steps):
         ctx.batch_multiply()
         ctx.batch_multiply(predictor=P)
