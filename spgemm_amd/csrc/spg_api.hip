@@ -204,7 +204,7 @@ int spgDestroy(spg_context* c) {
                       &c->bsplit,
                       &c->vub,
                       &c->vbase, &c->cursor, &c->kbin, &c->kperm, &c->kbin_off, &c->kitems,
-                      &c->kitem_off, &c->inter, &c->bpair, &c->vclass, &c->vlist,
+                      &c->kitem_off, &c->inter, &c->bpair, &c->ebase, &c->vclass, &c->vlist,
                       &c->vclass_off, &c->vcnt, &c->vout, &c->crow, &c->ccol, &c->cval,
                       &c->pred, &c->vcap, &c->pbase, &c->fbase, &c->pcol, &c->pval,
                       &c->fcol, &c->fval, &c->rwinb, &c->kwinb, &c->scratch};

@@ -1680,6 +1680,115 @@ __global__ void k_interleave(const int* __restrict__ idx, const float* __restric
     if (q < n) out[q] = make_int2(idx[q], __float_as_int(val[q]));
 }
 
+// Untiled rows, by chunks of kBlock consecutive A entries (in row order):
+// the slot of each entry's first product is known from its row's base and
+// the lengths of the row's earlier B rows, so a chunk's products form runs
+// of consecutive slots (consecutive untiled rows have adjacent slot ranges)
+// that each wave writes 64 slots per store, lanes finding their entry by a
+// binary search over the chunk's product offsets, as the row-chunk kernel
+// does per tile.  One workgroup per chunk amortizes the per-row chain of
+// dependent loads that a wave per row pays.
+__global__ void k_entry_base(int m, const int* __restrict__ rptr, const int* __restrict__ rk,
+                             const int* __restrict__ b_ptr, const int* __restrict__ vstart,
+                             const long long* __restrict__ vbase, long long* __restrict__ ebase) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int vs = vstart[i];
+    const bool untiled = vstart[i + 1] - vs == 1;
+    long long run = untiled ? vbase[vs] : -1;
+    const int e1 = rptr[i + 1];
+    for (int e = rptr[i]; e < e1; ++e) {
+        ebase[e] = run;
+        if (untiled) {
+            const int k = rk[e];
+            run += b_ptr[k + 1] - b_ptr[k];
+        }
+    }
+}
+
+struct EntryArgs {
+    const int* rk;
+    const float* rv;
+    const int* b_ptr;
+    const int2* bpair;
+    const long long* ebase;
+    int* icol;
+    float* ival;
+};
+
+__global__ __launch_bounds__(kBlock) void k_expand_entrychunks(EntryArgs g, long long nent) {
+    __shared__ int xs[kBlock];      // B position of each entry's row
+    __shared__ int pref[kBlock];    // each entry's offset in the chunk's products
+    __shared__ long long sbo[kBlock];  // slot of product offset 0 for the entry
+    __shared__ float sa[kBlock];
+    __shared__ int sh[kBlock / 64];
+    constexpr int NW = kBlock / 64;
+    const int wv = threadIdx.x >> 6, lane = lane_id();
+    const int2* __restrict__ bpair = g.bpair;
+    int* __restrict__ icol = g.icol;
+    float* __restrict__ ival = g.ival;
+    const long long nunits = (nent + kBlock - 1) / kBlock;
+    for (long long u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const long long c0 = u * kBlock;
+        const int ne = (int)min((long long)kBlock, nent - c0);
+        int len = 0, x = 0;
+        long long base = -1;
+        float a = 0.f;
+        if (threadIdx.x < ne) {
+            const long long e = c0 + threadIdx.x;
+            base = g.ebase[e];
+            if (base >= 0) {  // an entry of an untiled row
+                const int k = g.rk[e];
+                x = g.b_ptr[k];
+                len = g.b_ptr[k + 1] - x;
+                a = g.rv[e];
+            }
+        }
+        int sum;
+        const int off = block_exscan<int>(len, &sum, sh);  // also: the previous chunk is done
+        if (threadIdx.x < ne) {
+            xs[threadIdx.x] = x;
+            pref[threadIdx.x] = off;
+            sbo[threadIdx.x] = base - off;
+            sa[threadIdx.x] = a;
+        }
+        __syncthreads();
+        const int nbk = (sum + 63) >> 6;
+        const int per = (nbk + NW - 1) / NW;
+        const int pb = min(nbk, wv * per), pe = min(nbk, pb + per);
+        for (int p0 = pb; p0 < pe; p0 += 4) {
+            int pos[4], ev[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                pos[v] = min((min(p0 + v, pe - 1) << 6) + lane, sum - 1);
+                ev[v] = 0;  // largest entry whose offset is <= pos
+            }
+#pragma unroll
+            for (int step = kBlock / 2; step > 0; step >>= 1) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int cand = ev[v] + step;
+                    if (cand < ne && pref[cand] <= pos[v]) ev[v] = cand;
+                }
+            }
+            int col[4];
+            float bv[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int2 pr = bpair[xs[ev[v]] + pos[v] - pref[ev[v]]];
+                col[v] = pr.x;
+                bv[v] = __int_as_float(pr.y);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const long long w = islot(sbo[ev[v]] + pos[v]);
+                icol[w] = col[v];
+                ival[w] = __fmul_rn(sa[ev[v]], bv[v]);
+            }
+        }
+    }
+}
+
 // Row-chunk classes: tiled rows with at most kRCMaxTiles tiles, by bound
 // (>= 2^20, >= 2^16, the rest), so that the heaviest rows start first.
 __global__ void k_rc_key(int m, const int* __restrict__ vstart, const long long* __restrict__ ub,
@@ -1802,13 +1911,11 @@ __global__ void k_vclass_expand(long long V, const long long* __restrict__ vub,
     long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     const long long u = vub[v];
-    if (rc) {
-        const int i = vrow_row[v];
-        const int Ti = vstart[i + 1] - vstart[i];
-        if (Ti > 1 && Ti <= kRCMaxTiles) {
-            cls[v] = -1;
-            return;
-        }
+    const int i = vrow_row[v];
+    const int Ti = vstart[i + 1] - vstart[i];
+    if (Ti == 1 || (rc && Ti <= kRCMaxTiles)) {  // entry chunks / row chunks
+        cls[v] = -1;
+        return;
     }
     cls[v] = u == 0 ? -1 : (u <= kRowsWaveMax ? 0 : 1);
 }
@@ -1844,8 +1951,31 @@ int launch_expand_rows(spg_context* c) {
     g.icol = inter_cols(c);
     g.ival = inter_vals(c);
     const int* vl = (const int*)c->vlist.p;
+    // untiled rows: the slot of every entry's first product
+    const long long nent = c->A.nnz;
+    SPG_TRY(ensure(c, c->ebase, sizeof(long long) * (size_t)std::max<long long>(nent, 1)));
+    if (c->m > 0) {
+        hipLaunchKernelGGL(k_entry_base, dim3((c->m + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                           c->stream, c->m, g.rptr, g.rk, g.b_ptr, g.vstart, g.vbase,
+                           (long long*)c->ebase.p);
+        SPG_HIP(hipGetLastError());
+    }
     tmark(c, SPG_PHASE_EXPAND, 0);
     SPG_TRY(launch_rowchunks(c, pl));
+    if (nent > 0) {
+        EntryArgs h{};
+        h.rk = g.rk;
+        h.rv = g.rv;
+        h.b_ptr = g.b_ptr;
+        h.bpair = g.bpair;
+        h.ebase = (const long long*)c->ebase.p;
+        h.icol = g.icol;
+        h.ival = g.ival;
+        const long long nunits = (nent + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL(k_expand_entrychunks, dim3(persistent_grid(c, nunits, 8)), dim3(kBlock), 0,
+                           c->stream, h, nent);
+        SPG_HIP(hipGetLastError());
+    }
     if (n1 > 0) {  // largest rows first
         hipLaunchKernelGGL(k_expand_rows_wg, dim3(persistent_grid(c, n1, 8)), dim3(kBlock), 0,
                            c->stream, g, vl + off_h[1], n1);

@@ -90,6 +90,9 @@ struct spg_context {
     // B's (column, value) pairs interleaved, for the row-chunk expansion's
     // gathers (one 8-byte load per product)
     spg::DevBuf bpair;
+    // slot of the first product of every A entry of an untiled row (row-wise
+    // expansion by entry chunks), -1 for the entries of tiled rows
+    spg::DevBuf ebase;
     // symbolic / merge
     spg::DevBuf vclass, vlist, vclass_off, vcnt, vout, crow, ccol, cval;
     // learned merge: row predictions, per-virtual-row capacities, padded
