@@ -1688,7 +1688,7 @@ __global__ void k_interleave(const int* __restrict__ idx, const float* __restric
 // binary search over the chunk's product offsets, as the row-chunk kernel
 // does per tile.  One workgroup per chunk amortizes the per-row chain of
 // dependent loads that a wave per row pays.
-__global__ void k_entry_base(int m, const int* __restrict__ rptr, const int* __restrict__ rk,
+__global__ void k_expand_entry_base(int m, const int* __restrict__ rptr, const int* __restrict__ rk,
                              const int* __restrict__ b_ptr, const int* __restrict__ vstart,
                              const long long* __restrict__ vbase, long long* __restrict__ ebase) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1954,13 +1954,13 @@ int launch_expand_rows(spg_context* c) {
     // untiled rows: the slot of every entry's first product
     const long long nent = c->A.nnz;
     SPG_TRY(ensure(c, c->ebase, sizeof(long long) * (size_t)std::max<long long>(nent, 1)));
+    tmark(c, SPG_PHASE_EXPAND, 0);
     if (c->m > 0) {
-        hipLaunchKernelGGL(k_entry_base, dim3((c->m + kBlock - 1) / kBlock), dim3(kBlock), 0,
+        hipLaunchKernelGGL(k_expand_entry_base, dim3((c->m + kBlock - 1) / kBlock), dim3(kBlock), 0,
                            c->stream, c->m, g.rptr, g.rk, g.b_ptr, g.vstart, g.vbase,
                            (long long*)c->ebase.p);
         SPG_HIP(hipGetLastError());
     }
-    tmark(c, SPG_PHASE_EXPAND, 0);
     SPG_TRY(launch_rowchunks(c, pl));
     if (nent > 0) {
         EntryArgs h{};
