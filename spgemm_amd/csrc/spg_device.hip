@@ -1689,20 +1689,32 @@ __global__ void k_interleave(const int* __restrict__ idx, const float* __restric
 // does per tile.  One workgroup per chunk amortizes the per-row chain of
 // dependent loads that a wave per row pays.
 __global__ void k_expand_entry_base(int m, const int* __restrict__ rptr, const int* __restrict__ rk,
-                             const int* __restrict__ b_ptr, const int* __restrict__ vstart,
-                             const long long* __restrict__ vbase, long long* __restrict__ ebase) {
+                                    const int* __restrict__ b_ptr, const int* __restrict__ vstart,
+                                    const long long* __restrict__ vbase,
+                                    long long* __restrict__ ebase) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const int vs = vstart[i];
-    const bool untiled = vstart[i + 1] - vs == 1;
-    long long run = untiled ? vbase[vs] : -1;
+    if (vstart[i + 1] - vs != 1) return;  // a tiled row: its entries keep -1
+    long long run = vbase[vs];
     const int e1 = rptr[i + 1];
-    for (int e = rptr[i]; e < e1; ++e) {
-        ebase[e] = run;
-        if (untiled) {
-            const int k = rk[e];
-            run += b_ptr[k + 1] - b_ptr[k];
+    int e = rptr[i];
+    for (; e + 4 <= e1; e += 4) {  // four entries' loads in flight
+        int k[4], l[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) k[u] = rk[e + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) l[u] = b_ptr[k[u] + 1] - b_ptr[k[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            ebase[e + u] = run;
+            run += l[u];
         }
+    }
+    for (; e < e1; ++e) {
+        const int k = rk[e];
+        ebase[e] = run;
+        run += b_ptr[k + 1] - b_ptr[k];
     }
 }
 
@@ -1955,7 +1967,8 @@ int launch_expand_rows(spg_context* c) {
     const long long nent = c->A.nnz;
     SPG_TRY(ensure(c, c->ebase, sizeof(long long) * (size_t)std::max<long long>(nent, 1)));
     tmark(c, SPG_PHASE_EXPAND, 0);
-    if (c->m > 0) {
+    if (c->m > 0 && nent > 0) {
+        SPG_TRY(fill<long long>(c, (long long*)c->ebase.p, -1LL, nent));
         hipLaunchKernelGGL(k_expand_entry_base, dim3((c->m + kBlock - 1) / kBlock), dim3(kBlock), 0,
                            c->stream, c->m, g.rptr, g.rk, g.b_ptr, g.vstart, g.vbase,
                            (long long*)c->ebase.p);
