@@ -179,6 +179,17 @@ def test_rowchunk_split(spg, orc, gpu_ctx, tile_log2, long_ub, s, rc_ub):
     compare(out, oracle_all(orc, A, sh, A, sh))
 
 
+def test_rows_mode_many_tiles(spg, orc, gpu_ctx):
+    """ROWS mode with 512 tiles per long row (more than the row-chunk kernel
+    takes): those rows go through the per-virtual-row kernels, the untiled
+    rows through the entry-chunk kernel."""
+    print("case rows mode, 512 tiles", flush=True)
+    A, sh = gen(spg, "rmat", s=14, ef=4, seed=13)
+    out = run_gpu(spg, gpu_ctx, A, sh, A, sh,
+                  cfg=dict(tile_log2=5, long_row_ub=16, expand_mode=spg.EXPAND_ROWS))
+    compare(out, oracle_all(orc, A, sh, A, sh))
+
+
 @pytest.mark.parametrize("tmode", [2, 1], ids=["sort", "atomic"])
 def test_transpose_modes(spg, orc, gpu_ctx, tmode):
     """InitGEMM's A-by-rows transpose: the LDS counting sort and the
