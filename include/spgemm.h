@@ -203,8 +203,11 @@ typedef struct spg_config {
 
 /* Expansion mappings of BinGEMM.  KBINS groups the (i,k) pairs by k as the
  * reference does (one work item per k-chunk, k binned by nnz(B row k);
- * src/mm.cu:288-327); ROWS groups them by C row (virtual row), so that each
- * row's slot range is written front to back with deterministic offsets. */
+ * src/mm.cu:288-327); ROWS groups them by C row: A's entries in row order,
+ * in chunks (tiled rows: one run of slots per chunk and tile, reserved with
+ * one atomic; untiled rows: slots known from the row bases), so that every
+ * store writes consecutive slots.  The order of the products inside a row
+ * is not fixed in either mapping (slot reservations follow the schedule). */
 #define SPG_EXPAND_KBINS 0
 #define SPG_EXPAND_ROWS 1
 
