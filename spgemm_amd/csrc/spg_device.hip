@@ -1506,10 +1506,11 @@ __global__ __launch_bounds__(kBlock) void k_expand_rows_wg(RowsArgs g, const int
 // runs of a row's chunks are reserved at nearly the same time, so its lines
 // are completed within microseconds, where the binned expansion's short
 // pieces leave partial lines written far apart in time (L2 write-back for
-// the 23.5 GB of intermediates of R-MAT scale 18: 24.3 GB here, 29.0 GB
-// binned); the price is that B is read once per product (from L2 or the
-// last-level cache) instead of once per work item, so the two mappings end
-// up equally fast there (9.6 ms).  Lanes past a run's end repeat its
+// the 23.5 GB of intermediates of R-MAT scale 18: 24.2 GB here, 29.0 GB
+// binned).  The price is that B is read once per product (8-byte pairs,
+// from L2 or the last-level cache: 25 GB of L2 misses there) instead of
+// once per work item: 9.8 ms on that product, against 12.6 ms for the
+// binned kernel on 64-product groups.  Lanes past a run's end repeat its
 // last product (same slot, same data), so no store is under a lane
 // condition.
 constexpr int kRCTab = 2048;      // (entry, tile) slot bases of one chunk
