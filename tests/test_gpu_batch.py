@@ -88,14 +88,16 @@ def test_batch_matches_oracle(spg, orc, gpu_ctx, cfg, mode):
         check(ctx.batch_get_c(b), oracle_c(orc, A, B))
 
 
-def test_batch_graph_replays(spg, orc, gpu_ctx):
+@pytest.mark.parametrize("mode", [0, 1], ids=["kbins", "rows"])
+def test_batch_graph_replays(spg, orc, gpu_ctx, mode):
     """Record + capture, then replays with eager work and synchronizations
-    in between; every replay gives the recorded run's C."""
-    print("case batch graph", flush=True)
+    in between; every replay gives the recorded run's C (both expansion
+    mappings: the row-wise one adds its own readbacks, buffers and fills)."""
+    print(f"case batch graph mode={mode}", flush=True)
     pairs = make_batch(spg, 11, count=8, big=True)
     ctx = gpu_ctx
     ctx.set_config(tile_log2=13, item_products=4096, long_row_ub=1024, xcd_swizzle=1, timing=1,
-                   expand_mode=0)
+                   expand_mode=mode)
     ctx.batch_set_operands(pairs)
     ctx.batch_multiply(graph=True)
     first = [ctx.batch_get_c(b) for b in range(len(pairs))]
