@@ -5,6 +5,7 @@ the seeded generators behave as the reference's lib/coo.h + src/coo.cpp
 import ctypes
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -36,6 +37,39 @@ def test_struct_layouts(spg):
     assert ctypes.sizeof(spg.Triplet) == 12
     assert ctypes.sizeof(spg.COO) == 24
     assert ctypes.sizeof(spg.CM) == 40
+
+
+MIRRORS = [("Triplet", "triplet"), ("COO", "coo"), ("CM", "cm"), ("CSR64", "spg_csr64"),
+           ("Config", "spg_config"), ("LearnedStats", "spg_learned_stats")]
+
+
+def test_struct_mirrors_match_header(spg, tmp_path):
+    """Every ctypes mirror has the C struct's size and, field by field, its
+    offsets (a C program built from include/spgemm.h prints them): the
+    library writes whole structs into these mirrors (spgGetConfig,
+    spgGetLearnedStats, ...)."""
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "spgemm.h"', "int main(void) {"]
+    for py, c in MIRRORS:
+        lines.append(f'    printf("{py} size %zu\\n", sizeof({c}));')
+        for f, _ in getattr(spg, py)._fields_:
+            lines.append(f'    printf("{py} {f} %zu\\n", offsetof({c}, {f}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)],
+                   check=True, capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for ln in out:
+        if ln:
+            py, f, v = ln.split()
+            got[(py, f)] = int(v)
+    for py, _ in MIRRORS:
+        cls = getattr(spg, py)
+        assert got[(py, "size")] == ctypes.sizeof(cls), py
+        for f, _ in cls._fields_:
+            assert got[(py, f)] == getattr(cls, f).offset, (py, f)
 
 
 def load_mtx_expect(name):
