@@ -171,7 +171,7 @@ typedef struct spg_config {
     int64_t long_row_ub;       /* rows with more products are column-tiled     */
     int32_t xcd_swizzle;       /* 1: map consecutive work items to one XCD     */
     int32_t timing;            /* 1: record per-phase HIP events               */
-    int32_t expand_mode;       /* SPG_EXPAND_KBINS or SPG_EXPAND_ROWS          */
+    int32_t expand_mode;       /* SPG_EXPAND_ROWS (default) or SPG_EXPAND_KBINS */
     float pred_margin;         /* learned path: capacity = pred*(1+margin)+slack */
     int32_t pred_slack;        /* learned path: slots added to every capacity    */
     int32_t pred_tight;        /* learned path: 0 = capacity is the merge class's

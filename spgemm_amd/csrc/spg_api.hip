@@ -161,7 +161,7 @@ int spgCreate(spg_context** out, int device) {
     c->cfg.long_row_ub = kHashMaxCount;
     c->cfg.xcd_swizzle = 1;
     c->cfg.timing = 1;
-    c->cfg.expand_mode = SPG_EXPAND_KBINS;
+    c->cfg.expand_mode = SPG_EXPAND_ROWS;
     c->cfg.pred_margin = 0.25f;
     c->cfg.pred_slack = 8;
     c->cfg.heavy_factor = 50;
