@@ -357,6 +357,60 @@ def bench_batch_only(args, pairs):
     ctx.close()
 
 
+def bench_mtx_list(args):
+    """The reference's benchmark sweep (bin/launch.sh: ./outer X.mtx X.mtx for
+    every matrix of a list): C = A*A for each Matrix Market file named in
+    the list file (one path per line; blank lines and # comments skipped),
+    read with readMTX (values kept; --compat-values forces 1.1 as the
+    reference does).  Prints one JSON line per matrix: nprod, nnz(C) and the
+    GFLOP/s of the plain and the learned path (operands resident, median of
+    --steps timed steps after --warmup)."""
+    from spgemm_amd import train_predictor as tp
+    with open(args.mtx_list) as f:
+        paths = [ln.strip() for ln in f if ln.strip() and not ln.lstrip().startswith("#")]
+    base = os.path.dirname(os.path.abspath(args.mtx_list))
+    ctx = spg.Context(0)
+    P, _ = tp.load_or_train(ctx)
+    n_ok = 0
+    for p in paths:
+        full = p if os.path.isabs(p) else os.path.join(base, p)
+        rec = {"sweep": "C=A*A", "matrix": p}
+        try:
+            A = spg.read_mtx(full, compat_values=args.compat_values)
+        except spg.SpgError as e:
+            rec["error"] = f"read failed: {e}"
+            print(json.dumps(rec), flush=True)
+            continue
+        rec.update(m=A.num_rows, n=A.num_cols, nnz_A=int(A.nnz))
+        if A.num_rows != A.num_cols:
+            rec["error"] = "not square: A*A undefined"
+            print(json.dumps(rec), flush=True)
+            continue
+        ctx.set_operands_from_coo(A, A)
+        times = {"plain": [], "learned": []}
+        for _ in range(args.warmup + args.steps):
+            for name in ("plain", "learned"):
+                t0 = time.perf_counter()
+                if name == "plain":
+                    nprod, nnz = run_step(ctx)
+                else:
+                    nprod, nnz_l, _ = run_step_learned(ctx, P, A.num_rows)
+                    assert nnz_l == nnz, "learned and plain nnz(C) differ"
+                ctx.synchronize()
+                times[name].append((time.perf_counter() - t0) * 1e3)
+        C = ctx.get_c()
+        rec.update(nprod=int(nprod), nnz_C=int(nnz),
+                   nnz_C_above_1e-4=int(np.count_nonzero(C[2] > 1e-4)))
+        for name, t in times.items():
+            ms = float(np.median(t[args.warmup:]))
+            rec[name] = {"ms": round(ms, 4), "gflops": round(2.0 * nprod / (ms * 1e-3) / 1e9, 3)}
+        print(json.dumps(rec), flush=True)
+        n_ok += 1
+    P.close()
+    ctx.close()
+    return 0 if n_ok == len(paths) else 1
+
+
 def copy_probe(mib):
     """One device-to-device copy of mib MiB (the HIP runtime's copy kernel):
     the known-size transfer that scales the FETCH_SIZE / WRITE_SIZE counters
@@ -403,6 +457,10 @@ def main():
                     help="only run a device-to-device copy of this size (counter calibration)")
     ap.add_argument("--batch-only", action="store_true",
                     help="batch workloads: skip the one-product-per-launch baseline (profiling)")
+    ap.add_argument("--mtx-list", default=None,
+                    help="sweep: C=A*A for every Matrix Market file named in this list file")
+    ap.add_argument("--compat-values", action="store_true",
+                    help="--mtx-list: every value read as 1.1, as the reference's readMTX")
     ap.add_argument("--cpu-products", type=float, default=3e9)
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
@@ -412,6 +470,8 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by "
               "C row blocks across GPUs but this build runs on a single MI355X.", file=sys.stderr)
         sys.exit(2)
+    if args.mtx_list:
+        sys.exit(bench_mtx_list(args))
 
     if WORKLOADS[args.workload]["kind"] == "batch":
         return bench_batch(args)

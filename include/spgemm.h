@@ -278,6 +278,13 @@ int spgGetIntermediateByRow(spg_context* ctx, int64_t* row_ptr,
 /* Milliseconds per phase (SPG_NUM_PHASES) of the last product, from HIP
  * events on the context stream; requires cfg.timing. */
 int spgGetPhaseTimes(spg_context* ctx, double* ms);
+/* The 32-bit limits of the expansion (defect D10 of src/mm.cu:173-180, whose
+ * int offsets wrap silently): SPG_OK when the largest virtual-row bound and
+ * 256 times min(longest B row, largest untiled-row bound) both stay below
+ * 2^31 - 64, else SPG_EOVERFLOW.  InitGEMM applies it to the maxima it
+ * reduces on the GPU and fails with SPG_EOVERFLOW before any expansion. */
+int spgSlotLimitStatus(int64_t max_vrow_bound, int64_t max_untiled_bound,
+                       int64_t max_b_row);
 
 /* ---------------- learned allocation (no reference counterpart) ---------
  * A 4-64-64-1 FP32 MLP predicts nnz(C row i) from four features of the

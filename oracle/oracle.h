@@ -52,6 +52,9 @@ typedef struct orc_triplet {
 #define ORC_FAULT_TRANSPOSE_B 2     /* uses B's column index as its row index */
 #define ORC_FAULT_OFF_BY_ONE_UB 3   /* upper bound counts b_len(k)+1          */
 #define ORC_FAULT_VALUE_SIGN 4      /* negates the product a*b                */
+#define ORC_FAULT_SORT_MINOR 5      /* sort_coo orders by minor before major  */
+#define ORC_FAULT_PTR_UNSHIFTED 6   /* coo2cm counts major i at ptr[i], not i+1 */
+#define ORC_FAULT_CSR_ROW_IDX 7     /* csc_to_csr stores the row, not k       */
 void orc_set_fault(int fault);
 int orc_get_fault(void);
 

@@ -16,7 +16,9 @@ LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
 REF_LIB_PATH = os.path.join(HERE, "_ref", "libspgref.so")
 
 NUM_KBINS = 9
-FAULT_NONE, FAULT_DROP_LAST_K, FAULT_TRANSPOSE_B, FAULT_OFF_BY_ONE_UB, FAULT_VALUE_SIGN = range(5)
+ORC_ROW_MAJOR, ORC_COL_MAJOR = 0, 1
+(FAULT_NONE, FAULT_DROP_LAST_K, FAULT_TRANSPOSE_B, FAULT_OFF_BY_ONE_UB, FAULT_VALUE_SIGN,
+ FAULT_SORT_MINOR, FAULT_PTR_UNSHIFTED, FAULT_CSR_ROW_IDX) = range(8)
 
 _lib = None
 

@@ -51,10 +51,11 @@ static int cmp_by_col(const void* a, const void* b) {
     return 0;
 }
 
-/* src/coo.cpp:73-79 */
+/* src/coo.cpp:73-79 (ORC_FAULT_SORT_MINOR swaps the comparators: minor
+ * index first) */
 void orc_sort_coo(int64_t nnz, orc_triplet* t, int type) {
-    qsort(t, (size_t)nnz, sizeof(orc_triplet),
-          type == ORC_ROW_MAJOR ? cmp_by_row : cmp_by_col);
+    const int by_row = (type == ORC_ROW_MAJOR) != (g_fault == ORC_FAULT_SORT_MINOR);
+    qsort(t, (size_t)nnz, sizeof(orc_triplet), by_row ? cmp_by_row : cmp_by_col);
 }
 
 /* src/mm.cu:43-63 (coo2cm: idx/val copied in COO order, ptr histogram)
@@ -67,7 +68,7 @@ void orc_coo2cm(int64_t nnz, const orc_triplet* t, int type, int32_t nmajor,
         int32_t minor = type == ORC_ROW_MAJOR ? t[e].cidx : t[e].ridx;
         idx[e] = minor;
         val[e] = t[e].val;
-        ptr[major + 1] += 1;
+        ptr[major + (g_fault == ORC_FAULT_PTR_UNSHIFTED ? 0 : 1)] += 1;
     }
     for (int32_t i = 0; i < nmajor; ++i) ptr[i + 1] += ptr[i];
 }
@@ -83,7 +84,7 @@ void orc_csc_to_csr(int32_t m, int32_t K, const int32_t* a_ptr,
     for (int32_t k = 0; k < K; ++k) {
         for (int32_t e = a_ptr[k]; e < a_ptr[k + 1]; ++e) {
             int32_t p = cur[a_idx[e]]++;
-            r_idx[p] = k;
+            r_idx[p] = g_fault == ORC_FAULT_CSR_ROW_IDX ? a_idx[e] : k;
             r_val[p] = a_val[e];
         }
     }

@@ -140,6 +140,7 @@ _SIGS = {
     "spgGetIntermediateByRow": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_int32),
                                         POINTER(c_float)]),
     "spgGetPhaseTimes": (c_int, [c_void_p, POINTER(c_double)]),
+    "spgSlotLimitStatus": (c_int, [c_int64, c_int64, c_int64]),
     "spgPredictorCreate": (c_int, [POINTER(c_void_p), c_int, c_uint64]),
     "spgPredictorDestroy": (c_int, [c_void_p]),
     "spgPredictorGetLastError": (c_char_p, [c_void_p]),
@@ -291,6 +292,12 @@ def read_mtx(path, compat_values=False):
     _check(lib().spgReadMTX(byref(c), path.encode(), MTX_COMPAT_VALUES if compat_values else 0),
            what=f"read_mtx {path}")
     return _coo_from_lib(c)
+
+
+def slot_limit_status(max_vrow_bound, max_untiled_bound, max_b_row):
+    """spgSlotLimitStatus: OK, or OVERFLOW when a count of the expansion
+    would not fit its 32-bit slot cursor or chunk offsets."""
+    return lib().spgSlotLimitStatus(int(max_vrow_bound), int(max_untiled_bound), int(max_b_row))
 
 
 def sort_coo(coo, type):

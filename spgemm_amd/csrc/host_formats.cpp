@@ -413,3 +413,21 @@ extern "C" int spgGenStencil27(coo* out, int nx, int ny, int nz, uint64_t seed) 
     out->type = ROW_MAJOR;
     return SPG_OK;
 }
+
+// The expansion kernels keep three counts in 32 bits: a virtual row's slot
+// cursor (every product of the row is reserved through it, so its bound must
+// fit), the run offsets of a row chunk inside one tile (at most the tile's
+// bound) and the product offsets of a chunk of 256 consecutive A entries of
+// untiled rows (each entry contributes its B row, which is at most the
+// longest B row and at most its untiled row's bound).  Each store position
+// is computed as (block << 6) + lane, so the limit keeps 64 spare.  The
+// reference's int offsets overflow silently instead (src/mm.cu:173-180).
+extern "C" int spgSlotLimitStatus(int64_t max_vrow_bound, int64_t max_untiled_bound,
+                                  int64_t max_b_row) {
+    const int64_t lim = (int64_t)INT32_MAX - 64;
+    if (max_vrow_bound < 0 || max_untiled_bound < 0 || max_b_row < 0) return SPG_EINVAL;
+    if (max_vrow_bound > lim) return SPG_EOVERFLOW;
+    const int64_t per_entry = std::min(max_b_row, max_untiled_bound);
+    if (per_entry > lim / 256) return SPG_EOVERFLOW;
+    return SPG_OK;
+}

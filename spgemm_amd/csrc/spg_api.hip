@@ -453,8 +453,19 @@ int spgInitGEMM(spg_context* c, cm* C, int64_t* nprod) {
         SPG_TRY(ensure(c, c->bsplit, sizeof(int32_t) * (size_t)c->K * (size_t)(c->T + 1)));
     SPG_TRY(scratch_prepare(c, std::max<long long>(c->V, c->K) + 1));
     SPG_TRY(launch_init_vrows(c));
-    int64_t np = 0;
-    SPG_TRY(read_scalar<int64_t>(c, (const int64_t*)c->vbase.p + c->V, &np));
+    unsigned long long* d_lim = (unsigned long long*)scratch_alloc(c, 4 * sizeof(uint64_t));
+    if (!d_lim) return fail(c, SPG_ENOMEM, "init scratch");
+    SPG_TRY(launch_limits(c, d_lim));
+    int64_t lim[4];
+    SPG_TRY(readback(c, d_lim, sizeof(lim), lim));
+    if (spgSlotLimitStatus(lim[0], lim[1], lim[2]) != SPG_OK)
+        return fail(c, SPG_EOVERFLOW,
+                    "a virtual row (bound " + std::to_string(lim[0]) + ", untiled " +
+                        std::to_string(lim[1]) + ") or an expansion chunk (longest B row " +
+                        std::to_string(lim[2]) +
+                        ") reaches 2^31 products; raise long_row_ub / lower tile_log2 or "
+                        "split the product");
+    const int64_t np = lim[3];
     c->nprod = np;
     SPG_TRY(ensure(c, c->inter, sizeof(int32_t) * inter_words(np)));
     tmark(c, SPG_PHASE_INIT, 1);

@@ -907,6 +907,49 @@ int launch_init_vrows(spg_context* c) {
     return fill<int>(c, (int*)c->cursor.p, 0, c->V);
 }
 
+// The sizes the expansion's 32-bit slot cursors and chunk offsets depend on
+// (checked on the host by spgSlotLimitStatus): out[0] the largest virtual-row
+// bound, out[1] the largest bound of an untiled row, out[2] the longest B
+// row; out[3] receives nprod (vbase[V]).  out[0..2] start at zero.
+__global__ __launch_bounds__(kBlock) void k_limits(long long V, const long long* __restrict__ vub,
+                                                   const int* __restrict__ vrow_row,
+                                                   const int* __restrict__ vstart, int K,
+                                                   const int* __restrict__ b_ptr,
+                                                   const long long* __restrict__ vbase,
+                                                   unsigned long long* __restrict__ out) {
+    long long mv = 0, mu = 0, mb = 0;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
+        const long long u = vub[v];
+        const int i = vrow_row[v];
+        mv = max(mv, u);
+        if (vstart[i + 1] - vstart[i] == 1) mu = max(mu, u);
+    }
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < K; k += stride)
+        mb = max(mb, (long long)(b_ptr[k + 1] - b_ptr[k]));
+    mv = wave_max(mv);
+    mu = wave_max(mu);
+    mb = wave_max(mb);
+    if (lane_id() == 0) {
+        if (mv) atomicMax(&out[0], (unsigned long long)mv);
+        if (mu) atomicMax(&out[1], (unsigned long long)mu);
+        if (mb) atomicMax(&out[2], (unsigned long long)mb);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[3] = (unsigned long long)vbase[V];
+}
+
+int launch_limits(spg_context* c, unsigned long long* d_out) {
+    SPG_TRY(fill<unsigned long long>(c, d_out, 0ull, 4));
+    const long long work = std::max<long long>(c->V, c->K);
+    const unsigned g = persistent_grid(c, (work + kBlock - 1) / kBlock, 4);
+    hipLaunchKernelGGL(k_limits, dim3(g), dim3(kBlock), 0, c->stream, c->V,
+                       (const long long*)c->vub.p, (const int*)c->vrow_row.p,
+                       (const int*)c->vstart.p, c->K, c->B.ptr, (const long long*)c->vbase.p,
+                       d_out);
+    SPG_HIP(hipGetLastError());
+    return SPG_OK;
+}
+
 // -------------------------------------------------------------- categorize
 // src/mm.cu:196-257 with D5 (one range), D6 (b_len >= 128 -> bin 8) and D7
 // (64-bit threshold) fixed.

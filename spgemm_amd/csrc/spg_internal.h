@@ -182,6 +182,9 @@ int stable_partition(spg_context* c, const int32_t* key, int64_t n, int nb,
 int launch_coo2cm(spg_context* c, const triplet* d_trip, int nnz, int type,
                   int nmajor, int* ptr, int* idx, float* val);
 int launch_init(spg_context* c);
+// Largest vrow bound, largest untiled-row bound, longest B row and nprod
+// into d_out[0..3] (after launch_init_vrows).
+int launch_limits(spg_context* c, unsigned long long* d_out);
 size_t transpose_scratch(const spg_context* c, int m, long long nnz);  // launch_init's transpose
 int launch_categorize(spg_context* c);
 int launch_expand_binned(spg_context* c);

@@ -33,5 +33,16 @@ def gpu_ctx(spg):
     if spg.device_count() < 1:
         pytest.fail("GPU test collected but no device is visible")
     ctx = spg.Context(0)
+    ctx.defaults = ctx.get_config()
     yield ctx
     ctx.close()
+
+
+@pytest.fixture(autouse=True)
+def _fresh_config(request):
+    """Every test that uses the shared context starts from a new context's
+    configuration, so no setting leaks from one test into the next."""
+    if "gpu_ctx" in request.fixturenames:
+        ctx = request.getfixturevalue("gpu_ctx")
+        ctx.set_config(**ctx.defaults)
+    yield

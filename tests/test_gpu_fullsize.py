@@ -85,6 +85,100 @@ def predictor(spg, ctx):
     return P
 
 
+def sample_rows(ub, long_row_ub, seed, n_top=50, n_edge=500, n_tiled=500, n_rand=1000):
+    """Rows to check against the oracle: the n_top largest bounds (hub rows,
+    split into the most products per tile), tiled rows with the smallest
+    bounds above the tiling threshold, random tiled rows and random rows."""
+    rng = np.random.default_rng(seed)
+    order = np.argsort(ub, kind="stable")
+    tiled = np.nonzero(ub > long_row_ub)[0]
+    pick = [order[-n_top:]]
+    if len(tiled):
+        pick.append(tiled[np.argsort(ub[tiled], kind="stable")[:n_edge]])
+        pick.append(rng.choice(tiled, min(n_tiled, len(tiled)), replace=False))
+    pick.append(rng.choice(len(ub), min(n_rand, len(ub)), replace=False))
+    return np.unique(np.concatenate(pick)).astype(np.int32)
+
+
+def check_rows_vs_oracle(orc, C, A_csr, B_csr, m, n, rows):
+    """Rows of C against the oracle's row-wise Gustavson with FP32 products
+    (src/mm.cu:366-404's merge restated): structure bit-exact, values within
+    VAL_RTOL of the FP64 sum rounded to FP32."""
+    ptr, idx, val = C
+    gp, gi, gv, _ = orc.gustavson(m, n, A_csr, B_csr, rows=rows, prod32=True)
+    cnt = (ptr[rows + 1] - ptr[rows]).astype(np.int64)
+    assert np.array_equal(cnt, np.diff(gp)), "row counts differ from the oracle"
+    sel = np.concatenate([np.arange(ptr[i], ptr[i + 1]) for i in rows]) if len(rows) else []
+    assert np.array_equal(idx[sel], gi), "columns differ from the oracle"
+    got = val[sel].astype(np.float64)
+    ref = gv.astype(np.float32).astype(np.float64)
+    err = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-30)
+    err[(np.abs(got) <= 1e-30) & (np.abs(ref) <= 1e-30)] = 0.0
+    assert err.max(initial=0.0) <= VAL_RTOL, float(err.max())
+    return int(cnt.sum())
+
+
+def coo_csr(M):
+    from helpers import csr_of
+    return csr_of((M.rows, M.cols, M.vals), (M.num_rows, M.num_cols))
+
+
+BENCH_CFG = dict(tile_log2=13, item_products=8192, long_row_ub=512, xcd_swizzle=1, timing=1,
+                 heavy_factor=50, rank_min_count=1024)
+
+
+def test_rmat_s18_bench_config_vs_oracle(spg, orc, gpu_ctx):
+    """R-MAT scale 18, edge factor 16 with the settings the benchmark times
+    (row-chunk expansion at 32 tiles per long row, long_row_ub 512,
+    rank_min_count 1024): about 2k rows of the plain and of the learned C
+    against the oracle, the hub rows (up to 2.5M products, 32 tiles) among
+    them."""
+    print("case full R-MAT s18 ef16, benchmark settings, rows vs oracle", flush=True)
+    ctx = gpu_ctx
+    A = spg.gen_rmat(18, 18, 16 << 18, seed=0)
+    ctx.set_config(expand_mode=spg.EXPAND_ROWS, **BENCH_CFG)
+    ctx.set_operands_from_coo(A, A)
+    ctx.multiply()
+    ub = ctx.upper_bound(A.num_rows)
+    Cp = ctx.get_c()
+    check_structure(Cp, ub, A.num_cols, nprod_of(A, A))
+    Acsr = coo_csr(A)
+    rows = sample_rows(ub, BENCH_CFG["long_row_ub"], 0)
+    nchk = check_rows_vs_oracle(orc, Cp, Acsr, Acsr, A.num_rows, A.num_cols, rows)
+    print(f"  plain: {len(rows)} rows, {nchk} entries match the oracle")
+    P = predictor(spg, ctx)
+    try:
+        st = ctx.multiply_learned(P)
+        Cl = ctx.get_c()
+    finally:
+        P.close()
+    assert st["unrecovered"] == 0
+    same_c(Cl, Cp)
+    check_rows_vs_oracle(orc, Cl, Acsr, Acsr, A.num_rows, A.num_cols, rows)
+
+
+def test_rect_learned_rows_vs_oracle(spg, orc, gpu_ctx):
+    """The rectangular product through the learned path with the row-chunk
+    expansion: sampled rows against the oracle."""
+    print("case full rectangular R-MAT, learned, rows vs oracle", flush=True)
+    ctx = gpu_ctx
+    A = spg.gen_rmat(20, 18, 8 << 20, seed=4)
+    B = spg.gen_rmat(18, 18, 8 << 18, seed=5)
+    ctx.set_config(expand_mode=spg.EXPAND_ROWS, **BENCH_CFG)
+    ctx.set_operands_from_coo(A, B)
+    P = predictor(spg, ctx)
+    try:
+        st = ctx.multiply_learned(P)
+        Cl = ctx.get_c()
+    finally:
+        P.close()
+    assert st["unrecovered"] == 0
+    ub = ctx.upper_bound(A.num_rows)
+    check_structure(Cl, ub, B.num_cols, nprod_of(A, B))
+    rows = sample_rows(ub, BENCH_CFG["long_row_ub"], 1)
+    check_rows_vs_oracle(orc, Cl, coo_csr(A), coo_csr(B), A.num_rows, B.num_cols, rows)
+
+
 def test_rmat_s18_full(spg, gpu_ctx):
     """R-MAT scale 18, edge factor 16: plain and learned paths."""
     print("case full R-MAT s18 ef16", flush=True)
@@ -160,6 +254,44 @@ def test_rect_rmat_full(spg, gpu_ctx):
                  B.num_cols, 3)
     ctx.multiply()
     same_c(ctx.get_c(), Cl)
+
+
+def test_batch256_graph_products_vs_oracle(spg, orc, gpu_ctx):
+    """Products of the 256-product batch from a learned graph replay with
+    the row-chunk expansion, whole, against the oracle's restated method:
+    the three smallest, three mid-sized and the largest below 2^24
+    products."""
+    print("case full batch256, graph replay products vs oracle", flush=True)
+    import bench
+    from helpers import csc_of, csr_of
+    ctx = gpu_ctx
+    pairs = bench.make_batch(256, 5)
+    ctx.set_config(expand_mode=spg.EXPAND_ROWS, **BENCH_CFG)
+    ctx.batch_set_operands(pairs)
+    np_b = np.array([nprod_of(A, B) for A, B in pairs])
+    order = np.argsort(np_b, kind="stable")
+    small = [int(b) for b in order if np_b[b] < (1 << 24)]
+    pick = sorted(set(small[:3] + small[len(small) // 2 - 1:len(small) // 2 + 2] + small[-1:]))
+    P = predictor(spg, ctx)
+    try:
+        ctx.batch_multiply(predictor=P, graph=True)  # records and captures
+        st = ctx.batch_multiply(predictor=P, graph=True)  # replays
+        ctx.synchronize()
+    finally:
+        P.close()
+    assert st["unrecovered"] == 0
+    for b in pick:
+        A, B = pairs[b]
+        m, K, n = A.num_rows, A.num_cols, B.num_cols
+        Cg = ctx.batch_get_c(b)
+        ref, nprod = orc.spgemm_restated(m, n, K, csc_of((A.rows, A.cols, A.vals), (m, K)),
+                                         csr_of((B.rows, B.cols, B.vals), (K, n)))
+        assert nprod == np_b[b]
+        assert np.array_equal(Cg[0], ref[0]) and np.array_equal(Cg[1], ref[1]), f"product {b}"
+        err = np.abs(Cg[2].astype(np.float64) - ref[2]) / np.maximum(np.abs(ref[2]), 1e-30)
+        err[(np.abs(Cg[2]) <= 1e-30) & (np.abs(ref[2]) <= 1e-30)] = 0.0
+        assert err.max(initial=0.0) <= VAL_RTOL, (b, float(err.max()))
+    print(f"  products {pick} match the oracle")
 
 
 def test_batch256_full(spg, gpu_ctx):

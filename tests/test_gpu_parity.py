@@ -346,3 +346,31 @@ def test_wide_b_hash_path(spg, orc, gpu_ctx):
     B = (rb, cb, rng.uniform(-1, 1, len(rb)).astype(np.float32))
     compare(run_gpu(spg, gpu_ctx, A, (m, K), B, (K, n), via_coo=False),
             oracle_all(orc, A, (m, K), B, (K, n)))
+
+
+@pytest.mark.parametrize("rank_min", [128, 1024, 4096])
+def test_wide_row_merge_classes(spg, orc, gpu_ctx, rank_min):
+    """Untiled rows over a 2^17-column window (wider than a tile) with about
+    200 to 4000 distinct columns.  rank_min_count 128 merges them all through
+    the bitmap rank kernel; 1024 sends those up to 1024 columns to the LDS
+    hashes (512 to 2048 slots) and the rest to rank; 4096 sends every one to
+    a hash (up to 8192 slots)."""
+    print(f"case wide-row merge classes rank_min_count={rank_min}", flush=True)
+    rng = np.random.default_rng(11)
+    K, n, per_b = 4000, 1 << 17, 100
+    sizes = [2, 3, 5, 8, 12, 20, 30, 40]  # entries per row: 200 .. 4000 products
+    m = 8 * 40
+    ra, ca = [], []
+    for i in range(m):
+        r = sizes[i % len(sizes)]
+        ra += [i] * r
+        ca += list(rng.choice(K, r, replace=False))
+    ra, ca = np.array(ra, np.int32), np.array(ca, np.int32)
+    rb = np.repeat(np.arange(K), per_b).astype(np.int32)
+    cb = np.concatenate([rng.choice(n, per_b, replace=False) for _ in range(K)]).astype(np.int32)
+    A = (ra, ca, rng.uniform(-1, 1, len(ra)).astype(np.float32))
+    B = (rb, cb, rng.uniform(-1, 1, len(rb)).astype(np.float32))
+    out = run_gpu(spg, gpu_ctx, A, (m, K), B, (K, n), via_coo=False,
+                  cfg=dict(long_row_ub=4096, rank_min_count=rank_min))
+    assert int(np.diff(out["C"][0]).max()) > 3000 and int(np.diff(out["C"][0]).min()) < 250
+    compare(out, oracle_all(orc, A, (m, K), B, (K, n)))

@@ -212,3 +212,39 @@ def test_reference_loader_parity(spg, tmp_path):
             ours = spg.sort_coo(spg.read_mtx(path, compat_values=True), t)
             got = list(zip(ours.rows.tolist(), ours.cols.tolist(), ours.vals.tolist()))
             assert [(a, b, np.float32(v)) for a, b, v in ref] == [(a, b, np.float32(v)) for a, b, v in got]
+
+
+def test_slot_limit_status(spg):
+    """The 32-bit expansion limits become SPG_EOVERFLOW instead of wrapping
+    (src/mm.cu:173-180): synthetic virtual-row bounds reduced on the host the
+    way InitGEMM reduces them on the GPU, just below and at each limit."""
+    OK, OVER, INVAL = 0, 7, 1
+    lim = 2 ** 31 - 1 - 64
+    rng = np.random.default_rng(0)
+
+    def status(vub, untiled, blen):
+        vub = np.asarray(vub, np.int64)
+        untiled = np.asarray(untiled, bool)
+        mu = int(vub[untiled].max(initial=0))
+        return spg.slot_limit_status(int(vub.max(initial=0)), mu, int(np.max(blen, initial=0)))
+
+    vub = rng.integers(0, 1 << 20, 1000)
+    untiled = rng.random(1000) < 0.5
+    blen = rng.integers(0, 5000, 300)
+    assert status(vub, untiled, blen) == OK
+    # one tiled virtual row at the cursor limit, then one past it
+    v2 = vub.copy()
+    v2[np.nonzero(~untiled)[0][0]] = lim
+    assert status(v2, untiled, blen) == OK
+    v2[np.nonzero(~untiled)[0][0]] = lim + 1
+    assert status(v2, untiled, blen) == OVER
+    v2[np.nonzero(~untiled)[0][0]] = 3 * 2 ** 31
+    assert status(v2, untiled, blen) == OVER
+    # 256-entry chunks of untiled rows: bounded by min(longest B row, row bound)
+    per = lim // 256
+    v3 = vub.copy()
+    v3[np.nonzero(untiled)[0][0]] = per + 1
+    assert status(v3, untiled, [per]) == OK          # B rows short enough
+    assert status(v3, untiled, [per + 1]) == OVER    # both past: a chunk could wrap
+    assert status(vub, untiled, [10 ** 9]) == OK     # long B row, small untiled rows
+    assert spg.slot_limit_status(-1, 0, 0) == INVAL

@@ -254,3 +254,102 @@ def test_mlp_features_match_row_loop():
         exp = np.log1p([len(bs), sum(bs), max(bs), np.std(bs)])
         assert np.allclose(x[i], exp, rtol=1e-12, atol=1e-12)
         assert ub[i] == sum(bs)
+
+
+# ------------------------------------------------------ COO helpers (a3, a5)
+def _coo_with_ties(seed, nrows=60, ncols=45, nnz=3000):
+    """Random COO with repeated (row, col) pairs, exact duplicate triplets,
+    ties in value, and empty rows and columns (the last 5 of each)."""
+    rng = np.random.default_rng(seed)
+    r = rng.integers(0, nrows - 5, nnz).astype(np.int32)
+    c = rng.integers(0, ncols - 5, nnz).astype(np.int32)
+    v = rng.integers(-3, 4, nnz).astype(np.float32) * 0.5
+    r[-40:], c[-40:], v[-40:] = r[:40], c[:40], v[:40]  # exact duplicates
+    return r, c, v, nrows, ncols
+
+
+def _lexsorted(r, c, v, by_row):
+    major, minor = (r, c) if by_row else (c, r)
+    o = np.lexsort((v, minor, major))
+    return r[o], c[o], v[o]
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+@pytest.mark.parametrize("by_row", [True, False], ids=["row_major", "col_major"])
+def test_sort_coo_matches_lexsort(orc, seed, by_row):
+    """src/coo.cpp:43-79: (major, minor, value) order, against numpy's
+    lexsort; the sorted triplets are the input's as a multiset."""
+    r, c, v, _, _ = _coo_with_ties(seed)
+    t = orc.ORC_ROW_MAJOR if by_row else orc.ORC_COL_MAJOR
+    got = orc.sort_coo(r, c, v, t)
+    exp = _lexsorted(r, c, v, by_row)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+
+
+@pytest.mark.parametrize("by_row", [True, False], ids=["row_major", "col_major"])
+def test_coo2cm_matches_bincount(orc, by_row):
+    """src/mm.cu:43-63,100-108: ptr = exclusive scan of the major-index
+    histogram (empty majors included), idx/val in the sorted COO's order."""
+    r, c, v, nrows, ncols = _coo_with_ties(2)
+    r, c, v = _lexsorted(r, c, v, by_row)
+    nmajor = nrows if by_row else ncols
+    major, minor = (r, c) if by_row else (c, r)
+    t = orc.ORC_ROW_MAJOR if by_row else orc.ORC_COL_MAJOR
+    ptr, idx, val = orc.coo2cm(r, c, v, t, nmajor)
+    exp_ptr = np.concatenate([[0], np.cumsum(np.bincount(major, minlength=nmajor))])
+    assert np.array_equal(ptr, exp_ptr)
+    assert ptr[-1] == len(r) and np.all(np.diff(ptr)[-5:] == 0)
+    assert np.array_equal(idx, minor) and np.array_equal(val, v)
+    # (ptr, idx, val) read back by scipy is the triplets' matrix
+    M = (sp.csr_matrix if by_row else sp.csc_matrix)(
+        (val.astype(np.float64), idx, ptr), shape=(nrows, ncols))
+    dense = np.zeros((nrows, ncols))
+    np.add.at(dense, (r, c), v.astype(np.float64))
+    assert np.array_equal(M.toarray(), dense)
+
+
+def test_csc_to_csr_matches_scipy(orc):
+    """CSC -> CSR with k ascending inside every row (duplicates kept), as
+    scipy's tocsr() of a CSC matrix orders them."""
+    r, c, v, m, K = _coo_with_ties(3)
+    a_ptr, a_idx, a_val = csc_of((r, c, v), (m, K))
+    got_ptr, got_idx, got_val = orc.csc_to_csr(m, K, a_ptr, a_idx, a_val)
+    S = sp.csc_matrix((a_val, a_idx, a_ptr), shape=(m, K)).tocsr()
+    assert np.array_equal(got_ptr, S.indptr) and np.array_equal(got_idx, S.indices)
+    assert np.array_equal(got_val, S.data)
+    assert np.all(np.diff(got_ptr)[-5:] == 0)
+    # and as the test helper's independent lexsort build
+    h_ptr, h_idx, _ = csr_of((r, c, v), (m, K))
+    assert np.array_equal(got_ptr, h_ptr) and np.array_equal(got_idx, h_idx)
+
+
+def _coo_pins_hold(orc):
+    ok = True
+    for by_row in (True, False):
+        r, c, v, nrows, ncols = _coo_with_ties(4)
+        t = orc.ORC_ROW_MAJOR if by_row else orc.ORC_COL_MAJOR
+        got = orc.sort_coo(r, c, v, t)
+        ok &= all(np.array_equal(g, e) for g, e in zip(got, _lexsorted(r, c, v, by_row)))
+        rs, cs, vs = _lexsorted(r, c, v, by_row)
+        nmajor = nrows if by_row else ncols
+        ptr, idx, _ = orc.coo2cm(rs, cs, vs, t, nmajor)
+        major = rs if by_row else cs
+        ok &= np.array_equal(ptr, np.concatenate([[0], np.cumsum(np.bincount(major, minlength=nmajor))]))
+    a_ptr, a_idx, a_val = csc_of((r, c, v), (nrows, ncols))
+    gp, gi, _ = orc.csc_to_csr(nrows, ncols, a_ptr, a_idx, a_val)
+    S = sp.csc_matrix((a_val, a_idx, a_ptr), shape=(nrows, ncols)).tocsr()
+    ok &= np.array_equal(gp, S.indptr) and np.array_equal(gi, S.indices)
+    return bool(ok)
+
+
+@pytest.mark.parametrize("fault", [5, 6, 7], ids=["sort_minor", "ptr_unshifted", "csr_row_idx"])
+def test_coo_helper_faults_are_caught(orc, fault):
+    """A planted mistake in sort_coo, coo2cm or csc_to_csr fails its pin."""
+    assert _coo_pins_hold(orc)
+    try:
+        orc.set_fault(fault)
+        caught = not _coo_pins_hold(orc)
+    finally:
+        orc.set_fault(orc.FAULT_NONE)
+    assert caught
