@@ -148,9 +148,46 @@ def load_traffic(workload, path):
         return None
 
 
-def cpu_baseline(A, B, target_products, threads):
-    """Times the oracle's restated method on row blocks of the same product
-    (beginning, middle and end of the row range) until target_products."""
+def cpu_threads_available():
+    """Host threads the CPU baseline may use: the CPUs this process may run
+    on, at most 16 (one GPU's share of the host)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def _median_runs(run, runs=3):
+    """1 untimed warm-up, then the median wall time (s) of runs timed runs."""
+    run()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        run()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def _row_blocks(base, m, products):
+    """Three disjoint row blocks (start, middle, 90 %) of about products/3
+    intermediate products each."""
+    per = max(1, int(products) // 3)
+    starts = (0, m // 2, (9 * m) // 10)
+    blocks = []
+    for q, start in enumerate(starts):
+        limit = starts[q + 1] if q + 1 < len(starts) else m
+        e = int(np.searchsorted(base, base[start] + per, side="left"))
+        blocks.append((start, min(limit, max(start + 1, e))))
+    return blocks
+
+
+def cpu_baseline(A, B, threads, seconds_per_run=3.0):
+    """The oracle's restatement of the reference method (upper bound, k-bins,
+    expansion, merge, compaction) on row blocks of the same product, timed
+    on the host: single-threaded and with `threads` threads, each as 1
+    warm-up + the median of 3 runs of a sample sized to about
+    seconds_per_run (a calibration run on a small block fixes the size)."""
     from oracle import oracle as O
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from helpers import csc_of, csr_of
@@ -159,74 +196,82 @@ def cpu_baseline(A, B, target_products, threads):
     Acsc = csc_of((A.rows, A.cols, A.vals), (m, K))
     Bcsr = csr_of((B.rows, B.cols, B.vals), (K, n))
     ub, base, nprod = O.upper_bound(m, K, Acsc[0], Acsc[1], Bcsr[0])
-    blocks = []
-    if nprod <= target_products:
-        blocks = [(0, m)]  # the whole product, run until about 2 s
-    else:  # three disjoint row blocks of about target/3 products each
-        per = max(1, target_products // 3)
-        starts = (0, m // 2, (9 * m) // 10)
-        for q, start in enumerate(starts):
-            limit = starts[q + 1] if q + 1 < len(starts) else m
-            s = int(np.searchsorted(base, base[start] + per, side="left"))
-            blocks.append((start, min(limit, max(start + 1, s))))
-    tot_prod, tot_s, runs = 0, 0.0, 0
-    O.cpu_baseline(m, n, K, Acsc, Bcsr, blocks[0][0], min(blocks[0][1], blocks[0][0] + 8), threads)
-    while True:
-        for (r0, r1) in blocks:
-            t0 = time.perf_counter()
-            r = O.cpu_baseline(m, n, K, Acsc, Bcsr, r0, r1, threads)
-            tot_s += time.perf_counter() - t0
-            tot_prod += r["nprod"]
-        runs += 1
-        if len(blocks) > 1 or tot_s >= 2.0 or runs >= 200:
-            break
-    if len(blocks) == 1:
-        desc = f"the whole product, {runs} run(s)"
-    else:
-        desc = "rows " + ", ".join(f"[{a},{b})" for a, b in blocks) + " of the same product"
-    return {"value": round(2.0 * tot_prod / tot_s / 1e9, 4), "unit": "GFLOP/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{desc}: {tot_prod} intermediate products ({nprod} in one product), "
-                      f"restated method "
-                      f"(upper bound, k-bins, expansion, merge, compaction), {tot_s:.1f} s"}
+    res = {}
+    for nt in sorted({1, threads}):
+        calib = _row_blocks(base, m, min(nprod, 2e7 * nt))
+        t0 = time.perf_counter()
+        got = sum(O.cpu_baseline(m, n, K, Acsc, Bcsr, a, b, nt)["nprod"] for a, b in calib)
+        rate = max(got, 1) / max(time.perf_counter() - t0, 1e-6)
+        blocks = _row_blocks(base, m, min(nprod, rate * seconds_per_run))
+        prods = [0]
+
+        def run():
+            prods[0] = sum(O.cpu_baseline(m, n, K, Acsc, Bcsr, a, b, nt)["nprod"] for a, b in blocks)
+
+        sec = _median_runs(run)
+        res[nt] = {"value": round(2.0 * prods[0] / sec / 1e9, 4), "sec": round(sec, 3),
+                   "products": int(prods[0]), "blocks": blocks}
+    def desc(r):
+        return ("rows " + ", ".join(f"[{a},{b})" for a, b in r["blocks"]) +
+                f" of the same product ({r['products']} of its {nprod} intermediate products), "
+                f"restated method (upper bound, k-bins, expansion, merge, compaction), "
+                f"1 warm-up + median of 3 runs ({r['sec']} s each)")
+
+    top = res[threads]
+    return {"value": top["value"], "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "sample": desc(top), "cores_visible": os.cpu_count(),
+            "single_thread": {"value": res[1]["value"], "unit": "GFLOP/s", "cores": 1,
+                              "sample": desc(res[1])}}
 
 
-def cpu_baseline_batch(pairs, target_products, threads):
-    """Times the oracle's restated method on whole products of the batch
-    (seeded random order) until target_products."""
+def cpu_baseline_batch(pairs, threads, seconds_per_run=2.0):
+    """The restated method on whole products of the batch (seeded random
+    order, as many as take about seconds_per_run), 1 warm-up + median of 3,
+    with `threads` threads."""
     from oracle import oracle as O
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from helpers import csc_of, csr_of
     order = np.random.default_rng(1).permutation(len(pairs))
-    tot_prod, tot_s, used = 0, 0.0, []
+    ops = []
+    spent, total = 0.0, 0
     for b in order:
         A, B = pairs[b]
         m, K, n = A.num_rows, A.num_cols, B.num_cols
-        Acsc = csc_of((A.rows, A.cols, A.vals), (m, K))
-        Bcsr = csr_of((B.rows, B.cols, B.vals), (K, n))
+        op = (m, n, K, csc_of((A.rows, A.cols, A.vals), (m, K)), csr_of((B.rows, B.cols, B.vals), (K, n)))
         t0 = time.perf_counter()
-        r = O.cpu_baseline(m, n, K, Acsc, Bcsr, 0, m, threads)
-        tot_s += time.perf_counter() - t0
-        tot_prod += r["nprod"]
-        used.append(int(b))
-        if tot_prod >= target_products:
+        total += O.cpu_baseline(*op, 0, m, threads)["nprod"]
+        spent += time.perf_counter() - t0
+        ops.append((int(b), op))
+        if spent >= seconds_per_run:
             break
-    return {"value": round(2.0 * tot_prod / tot_s / 1e9, 4), "unit": "GFLOP/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{len(used)} of {len(pairs)} products of the batch (seeded order), "
-                      f"{tot_prod} intermediate products, restated method (upper bound, k-bins, "
-                      f"expansion, merge, compaction), {tot_s:.1f} s"}
+
+    def run():
+        for _, op in ops:
+            O.cpu_baseline(*op, 0, op[0], threads)
+
+    sec = _median_runs(run)
+    return {"value": round(2.0 * total / sec / 1e9, 4), "unit": "GFLOP/s", "cores": threads,
+            "kind": "port", "cores_visible": os.cpu_count(),
+            "sample": f"{len(ops)} of {len(pairs)} products of the batch (seeded order), "
+                      f"{total} intermediate products, restated method (upper bound, k-bins, "
+                      f"expansion, merge, compaction), 1 warm-up + median of 3 runs ({sec:.2f} s)"}
 
 
 def bench_batch(args):
-    """256 uneven products: one product per launch sequence (plain path,
-    every product's operands resident on its own context) against the whole
-    batch in one replayed graph (plain, and with learned allocation)."""
-    from spgemm_amd import train_predictor as tp
+    """--workload batch256: the batch's own line."""
     w = WORKLOADS[args.workload]
     pairs = make_batch(w["count"], w["seed"])
     if args.batch_only:
         return bench_batch_only(args, pairs)
+    out = batch_result(args, pairs, "batch256")
+    print(json.dumps(out), flush=True)
+
+
+def batch_result(args, pairs, workload):
+    """256 uneven products: one product per launch sequence (plain path,
+    every product's operands resident on its own context) against the whole
+    batch in one replayed graph (plain, and with learned allocation)."""
+    from spgemm_amd import train_predictor as tp
     nprods = []
     ctxs = []
     cfg = dict(expand_mode=spg.EXPAND_KBINS)
@@ -309,18 +354,19 @@ def bench_batch(args):
                       "products": len(pairs), "nnz_A": int(nnzA), "nprod": nprod,
                       "nnz_C": nnz_total, "parallelism": "single GPU"},
            "phase_ms": {k: round(v, 4) for k, v in head["phase_ms"].items()},
-           "roofline": roofline_of(bytes_alg, head["phase_ms"], args.workload, "learned"),
+           "roofline": roofline_of(bytes_alg, head["phase_ms"], workload, "learned"),
            "plain_one_per_launch": {
                "value": round(plain_gfl, 2), "ms_per_batch": round(plain_ms, 4),
                "latency_ms_p50": round(float(np.percentile(per, 50)), 4),
                "latency_ms_p99": round(float(np.percentile(per, 99)), 4),
                "nprod_p50": int(np.percentile(nprods, 50)), "nprod_p99": int(np.percentile(nprods, 99))},
-           "learned_batched_vs_plain": round(gfl["batched_learned"] / plain_gfl, 4)}
+           "learned_batched_vs_plain": round(gfl["batched_learned"] / plain_gfl, 4),
+           "learned_batched_vs_batched_plain": round(gfl["batched_learned"] / gfl["batched_plain"], 4)}
     for k, r in res.items():
         out[k] = {"value": round(gfl[k], 2), "ms_per_batch": round(r["ms"], 4),
                   "latency_ms_every_product": round(r["ms"], 4),
                   "eager_phase_ms": {p: round(v, 4) for p, v in r["phase_ms"].items()},
-                  "roofline_frac": roofline_of(bytes_alg, r["phase_ms"], args.workload,
+                  "roofline_frac": roofline_of(bytes_alg, r["phase_ms"], workload,
                                                "learned")["frac"]}
         if r["tuned"]:
             out[k]["tuned_config"] = {f: r["tuned"][f] for f in TUNED_FIELDS}
@@ -332,9 +378,8 @@ def bench_batch(args):
         "unrecovered_overflows": int(st["unrecovered"])}
     out["batched_learned"]["predictor_training"] = train_info
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline_batch(pairs, int(args.cpu_products), threads)
-    print(json.dumps(out), flush=True)
+        out["cpu_baseline"] = cpu_baseline_batch(pairs, args.cpu_threads or cpu_threads_available())
+    return out
 
 
 def bench_batch_only(args, pairs):
@@ -399,11 +444,11 @@ def bench_mtx_list(args):
                 ctx.synchronize()
                 times[name].append((time.perf_counter() - t0) * 1e3)
         C = ctx.get_c()
-        rec.update(nprod=int(nprod), nnz_C=int(nnz),
-                   nnz_C_above_1e-4=int(np.count_nonzero(C[2] > 1e-4)))
+        rec.update(nprod=int(nprod), nnz_C=int(nnz))
+        rec["nnz_C_above_1e-4"] = int(np.count_nonzero(C[2] > 1e-4))  # src/mm.cu:438-449
         for name, t in times.items():
             ms = float(np.median(t[args.warmup:]))
-            rec[name] = {"ms": round(ms, 4), "gflops": round(2.0 * nprod / (ms * 1e-3) / 1e9, 3)}
+            rec[name] = {"ms": round(ms, 4), "gflops": float("%.4g" % (2.0 * nprod / (ms * 1e-3) / 1e9))}
         print(json.dumps(rec), flush=True)
         n_ok += 1
     P.close()
@@ -461,7 +506,8 @@ def main():
                     help="sweep: C=A*A for every Matrix Market file named in this list file")
     ap.add_argument("--compat-values", action="store_true",
                     help="--mtx-list: every value read as 1.1, as the reference's readMTX")
-    ap.add_argument("--cpu-products", type=float, default=3e9)
+    ap.add_argument("--no-batch", action="store_true",
+                    help="default workload: skip the attached 256-product batch measurement")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
     if args.copy_probe_mib:
@@ -550,11 +596,20 @@ def main():
         if cfg:
             out[k]["tuned_config"] = {f: cfg[f] for f in TUNED_FIELDS}
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(A, B, int(args.cpu_products), threads)
+        out["cpu_baseline"] = cpu_baseline(A, B, args.cpu_threads or cpu_threads_available())
     if P is not None:
         P.close()
     ctx.close()
+    if args.workload == "rmat_s18_ef16" and not args.no_batch:
+        # the batched half of the north-star ratio: the 256-product batch
+        # (learned + batched graph vs batched plain vs one product per launch)
+        w = WORKLOADS["batch256"]
+        b = batch_result(args, make_batch(w["count"], w["seed"]), "batch256")
+        out["batch256"] = {k: b[k] for k in ("value", "unit", "ms_per_step", "config", "phase_ms",
+                                             "roofline", "plain_one_per_launch", "batched_plain",
+                                             "batched_learned", "learned_batched_vs_plain",
+                                             "learned_batched_vs_batched_plain", "cpu_baseline")
+                           if k in b}
     print(json.dumps(out), flush=True)
 
 
