@@ -286,6 +286,30 @@ int spgGetPhaseTimes(spg_context* ctx, double* ms);
 int spgSlotLimitStatus(int64_t max_vrow_bound, int64_t max_untiled_bound,
                        int64_t max_b_row);
 
+/* ---------------- allocation-free product (no reference counterpart) ----
+ * For callers that cannot allocate device memory (serving loops and
+ * harnesses that forbid hipMalloc): spgMultiply's phases on the caller's
+ * device arrays, with every intermediate buffer taken from a fixed device
+ * array of SPG_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds
+ * until the process ends.  No hipMalloc, hipFree or pinned allocation is made
+ * (a failing call may build its message on the host heap).  A (m x K) is CSC
+ * (a_ptr has K+1 entries), B (K x n) is CSR (b_ptr has K+1 entries, columns
+ * ascending in each row).  C is written to c_ptr (m+1 int64), c_idx and c_val
+ * (c_capacity entries each); *c_nnz receives nnz(C).  Returns SPG_EOVERFLOW
+ * with *c_nnz = the required capacity when C does not fit (c_ptr then
+ * undefined), SPG_ENOMEM when the product needs more scratch than the
+ * maximum (roughly 8 bytes per intermediate product plus 1 KB per virtual
+ * row and 16 bytes per entry of A), and returns after the work has finished
+ * on stream (hipStream_t as void*, may be NULL).  Calls on one GPU share its
+ * scratch and race when they overlap (several host threads, or streams).
+ * err (may be NULL) receives the failure message. */
+#define SPG_NOALLOC_SCRATCH_BYTES (256u << 20)
+int spgMultiplyNoAlloc(int device, void* stream, const int* a_ptr, const int* a_idx,
+                       const float* a_val, int m, int K, int nnz_a, const int* b_ptr,
+                       const int* b_idx, const float* b_val, int n, int nnz_b,
+                       int64_t* c_ptr, int32_t* c_idx, float* c_val, int64_t c_capacity,
+                       int64_t* c_nnz, char* err, size_t err_len);
+
 /* ---------------- learned allocation (no reference counterpart) ---------
  * A 4-64-64-1 FP32 MLP predicts nnz(C row i) from four features of the
  * row: log1p of nnz(A row i), of ub_i, of max_k nnz(B row k) and of the

@@ -18,7 +18,7 @@ raises :class:`SpgError` if it is missing or fails.
 """
 import ctypes
 import os
-from ctypes import (POINTER, Structure, byref, c_bool, c_char_p, c_double, c_float,
+from ctypes import (POINTER, Structure, byref, c_bool, c_char_p, c_double, c_float, c_size_t,
                     c_int, c_int32, c_int64, c_uint64, c_void_p)
 
 import numpy as np
@@ -141,6 +141,10 @@ _SIGS = {
                                         POINTER(c_float)]),
     "spgGetPhaseTimes": (c_int, [c_void_p, POINTER(c_double)]),
     "spgSlotLimitStatus": (c_int, [c_int64, c_int64, c_int64]),
+    "spgMultiplyNoAlloc": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                                   c_int, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p,
+                                   c_void_p, c_void_p, c_int64, POINTER(c_int64), c_char_p,
+                                   c_size_t]),
     "spgPredictorCreate": (c_int, [POINTER(c_void_p), c_int, c_uint64]),
     "spgPredictorDestroy": (c_int, [c_void_p]),
     "spgPredictorGetLastError": (c_char_p, [c_void_p]),

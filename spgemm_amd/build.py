@@ -21,7 +21,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
           # packed FP32 instructions are kept out of every kernel
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["spg_device.hip", "spg_api.hip", "spg_predict.hip", "spg_batch.hip", "spg_tune.hip", "host_formats.cpp"]
+SOURCES = ["spg_device.hip", "spg_api.hip", "spg_predict.hip", "spg_batch.hip", "spg_tune.hip",
+           "spg_noalloc.hip", "host_formats.cpp"]
 
 
 def _compile(src):

@@ -24,8 +24,24 @@ int fail(spg_context* c, int code, const std::string& msg) {
 }
 
 int ensure(spg_context* c, DevBuf& b, size_t bytes) {
+    if (b.fixed) {
+        if (b.cap >= bytes) return SPG_OK;
+        return fail(c, SPG_EOVERFLOW, "C needs " + std::to_string(bytes) +
+                                          " bytes in an array the caller sized " + std::to_string(b.cap));
+    }
     if (bytes == 0) bytes = 8;
     if (b.cap >= bytes) return SPG_OK;
+    if (c->noalloc) {  // a new range of the arena (the old one is not reused)
+        const size_t off = (c->arena_top + 255) & ~(size_t)255;
+        if (off > c->arena_cap || bytes > c->arena_cap - off)
+            return fail(c, SPG_ENOMEM, "the product needs more than the " +
+                                           std::to_string(c->arena_cap >> 20) +
+                                           " MiB of the allocation-free scratch");
+        b.p = c->arena + off;
+        b.cap = bytes;
+        c->arena_top = off + bytes;
+        return SPG_OK;
+    }
     if (b.p) {
         SPG_HIP(hipStreamSynchronize(c->stream));
         SPG_HIP(hipFree(b.p));
@@ -49,9 +65,10 @@ int ensure(spg_context* c, DevBuf& b, size_t bytes) {
 }
 
 static void release(DevBuf& b) {
-    if (b.p) (void)hipFree(b.p);
+    if (b.p && !b.fixed) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
+    b.fixed = false;
 }
 
 // Scratch is a bump region sized before a phase starts (never reallocated
@@ -86,6 +103,12 @@ static int copy_d2h(spg_context* c, void* dst, const void* src, size_t bytes) {
 }
 
 int readback(spg_context* c, const void* d, size_t bytes, void* h) {
+    if (c->noalloc) {  // pageable staging: no pinned allocation
+        if (bytes > kReadbackMax) return fail(c, SPG_EINVAL, "readback too large");
+        SPG_TRY(copy_d2h(c, c->host_stage, d, bytes));
+        memcpy(h, c->host_stage, bytes);
+        return SPG_OK;
+    }
     if (bytes > c->pinned_small.cap) return fail(c, SPG_EINVAL, "readback too large");
     if (c->plan_mode == 2) {
         if (c->plan_pos + bytes > c->plan.size())
@@ -192,13 +215,13 @@ int spgCreate(spg_context** out, int device) {
     return SPG_OK;
 }
 
-int spgDestroy(spg_context* c) {
-    if (!c) return SPG_EINVAL;
-    (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    graph_release(c);
-    operand_release(c->A);
-    operand_release(c->B);
+}  // extern "C"
+
+namespace spg {
+
+// Every device buffer of a context (the operands' storage aside).
+template <typename F>
+static void for_each_buffer(spg_context* c, F f) {
     DevBuf* bufs[] = {&c->ub, &c->ntile, &c->vstart, &c->vrow_row, &c->vlo, &c->vwid,
                       &c->counters, &c->arow_ptr, &c->arow_k, &c->arow_v, &c->tilebits, &c->feat,
                       &c->bsplit,
@@ -208,7 +231,26 @@ int spgDestroy(spg_context* c) {
                       &c->vclass_off, &c->vcnt, &c->vout, &c->crow, &c->ccol, &c->cval,
                       &c->pred, &c->vcap, &c->pbase, &c->fbase, &c->pcol, &c->pval,
                       &c->fcol, &c->fval, &c->rwinb, &c->kwinb, &c->scratch};
-    for (DevBuf* b : bufs) release(*b);
+    for (DevBuf* b : bufs) f(*b);
+}
+
+void forget_buffers(spg_context* c) {
+    for_each_buffer(c, [](DevBuf& b) { b = DevBuf(); });
+    c->scratch_top = 0;
+}
+
+}  // namespace spg
+
+extern "C" {
+
+int spgDestroy(spg_context* c) {
+    if (!c) return SPG_EINVAL;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    graph_release(c);
+    operand_release(c->A);
+    operand_release(c->B);
+    for_each_buffer(c, [](DevBuf& b) { release(b); });
     if (c->pinned_small.p) (void)hipHostFree(c->pinned_small.p);
     for (auto& e : c->timer.ev)
         if (e) (void)hipEventDestroy(e);
@@ -544,8 +586,8 @@ int spgSymbolicGEMM(spg_context* c, int64_t* nnz) {
     SPG_TRY(read_scalar<int64_t>(c, (const int64_t*)c->vout.p + c->V, &total));
     c->nnzC = total;
     SPG_TRY(ensure(c, c->crow, sizeof(int64_t) * ((size_t)c->m + 1)));
-    SPG_TRY(ensure(c, c->ccol, sizeof(int32_t) * (size_t)(total + 1)));
-    SPG_TRY(ensure(c, c->cval, sizeof(float) * (size_t)(total + 1)));
+    SPG_TRY(ensure(c, c->ccol, sizeof(int32_t) * (size_t)total));
+    SPG_TRY(ensure(c, c->cval, sizeof(float) * (size_t)total));
     SPG_TRY(launch_rowptr(c));
     if (nnz) *nnz = total;
     c->stage = 4;

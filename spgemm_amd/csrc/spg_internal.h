@@ -22,10 +22,12 @@ constexpr int kHashMinLog2 = 6;         // smallest hash: 64 slots
 constexpr int kNumHashClasses = kHashMaxLog2 - kHashMinLog2 + 1;  // 64..8192
 constexpr int kNumVClasses = kNumHashClasses + 5;  // + tile (large), rank, tile small/medium, rank (wave)
 
-// Grow-only device buffer.
+// Grow-only device buffer.  fixed: memory the caller owns (an output of the
+// allocation-free entry point), never replaced; too small is an error.
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    bool fixed = false;
 };
 
 struct Operand {
@@ -107,6 +109,15 @@ struct spg_context {
     spg::Timer timer;
     double phase_ms[SPG_NUM_PHASES] = {};
 
+    // Allocation-free mode (spgMultiplyNoAlloc): every device buffer is a
+    // range of one fixed per-device array (arena, bump-allocated from
+    // arena_top and never freed), and readbacks go through host_stage
+    // (pageable, kReadbackMax bytes) instead of pinned memory.
+    bool noalloc = false;
+    char* arena = nullptr;
+    size_t arena_cap = 0, arena_top = 0;
+    unsigned char* host_stage = nullptr;
+
     // Replay log.  A recorded run appends every host readback (sizes, class
     // offsets) to plan; a replayed run (the one captured into a HIP graph)
     // takes them from plan instead of reading the device, so it contains no
@@ -153,6 +164,8 @@ inline float* inter_vals(spg_context* c) { return (float*)c->inter.p + kIGroup; 
 
 namespace spg {
 
+constexpr size_t kReadbackMax = 4096;  // bytes of one host readback
+
 // Device memory helpers (spg_api.hip).
 int ensure(spg_context* c, DevBuf& b, size_t bytes);
 void* scratch_alloc(spg_context* c, size_t bytes);
@@ -165,6 +178,9 @@ void tmark(spg_context* c, int phase, int end);
 int readback(spg_context* c, const void* d, size_t bytes, void* h);
 // Drops a captured batch graph (operands, config or predictor changed).
 void graph_release(spg_context* c);
+// Forgets every device buffer of the context without freeing it (the
+// allocation-free mode's buffers all live in its arena).
+void forget_buffers(spg_context* c);
 // Fill kernel (used instead of hipMemsetAsync so the path stays capturable).
 template <typename T>
 int fill(spg_context* c, T* p, T v, long long n);
