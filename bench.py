@@ -474,9 +474,22 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": n, "copies": 2}), flush=True)
 
 
+def parse_set(text):
+    """--set tile_log2=13,long_row_ub=512: fixed settings (spg_config fields)."""
+    out = {}
+    for item in filter(None, (text or "").split(",")):
+        k, v = item.split("=")
+        out[k.strip()] = float(v) if k.strip() == "pred_margin" else int(v)
+    return out
+
+
 def apply_tuning(ctx, args, predictor, learned):
     """Tunes the context for the path (untimed), or applies the entry of
-    --tune-table for this workload when there is one; returns the settings."""
+    --tune-table for this workload when there is one, or the fixed --set
+    settings; returns the settings."""
+    if args.set:
+        ctx.set_config(**parse_set(args.set))
+        return ctx.get_config()
     if args.no_tune:
         return None
     if args.tune_table and ctx.tune_lookup(args.tune_table, learned=learned):
@@ -496,6 +509,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (untimed) that picks each path's tunables first")
+    ap.add_argument("--set", default=None,
+                    help="fixed settings instead of tuning, e.g. tile_log2=13,long_row_ub=512")
     ap.add_argument("--tune-table", default=None,
                     help="JSON-lines table: apply its entry for the workload, else tune and append")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
