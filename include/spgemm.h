@@ -369,6 +369,7 @@ int spgMultiplyLearned(spg_context* ctx, const spg_predictor* p,
  * stacked block-diagonally (product b's columns start at a multiple of
  * 8192) and every phase kernel runs once over the work of all products. */
 #define SPG_BATCH_GRAPH 1  /* record the run once, then replay it as one HIP graph */
+#define SPG_BATCH_LATENCY 2  /* record every product's completion time on the GPU */
 
 /* A[b]: CSC (COL_MAJOR), B[b]: CSR (ROW_MAJOR), host arrays (copied). */
 int spgBatchSetOperands(spg_context* ctx, int count, const cm* A, const cm* B);
@@ -385,6 +386,12 @@ int spgBatchMultiply(spg_context* ctx, const spg_predictor* p, int flags,
                      spg_learned_stats* st);
 /* Host copy of product b's C (columns relative to its block). */
 int spgBatchGetC(spg_context* ctx, int b, spg_csr64* out);
+/* After a spgBatchMultiply with SPG_BATCH_LATENCY: ms[b] (count entries) =
+ * milliseconds from the start of the run to the moment the GPU wrote
+ * product b's last C row (device wall clock; the kernel writing a row's
+ * final entries stamps its product), 0 for a product without entries.
+ * Synchronizes the context stream. */
+int spgBatchGetLatency(spg_context* ctx, double* ms);
 
 /* ---------------- autotuner (no reference counterpart) ------------------
  * Coordinate descent over tile_log2, long_row_ub, item_products,

@@ -39,6 +39,7 @@ TRANSPOSE_AUTO = 0
 TRANSPOSE_ATOMIC = 1
 TRANSPOSE_SORT = 2
 BATCH_GRAPH = 1
+BATCH_LATENCY = 2
 
 STATUS = {0: "SPG_OK", 1: "SPG_EINVAL", 2: "SPG_ENOMEM", 3: "SPG_EHIP", 4: "SPG_EORDER",
           5: "SPG_EIO", 6: "SPG_ENODEV", 7: "SPG_EOVERFLOW"}
@@ -164,6 +165,7 @@ _SIGS = {
     "spgBatchGetLayout": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64)]),
     "spgBatchMultiply": (c_int, [c_void_p, c_void_p, c_int, POINTER(LearnedStats)]),
     "spgBatchGetC": (c_int, [c_void_p, c_int, POINTER(CSR64)]),
+    "spgBatchGetLatency": (c_int, [c_void_p, POINTER(c_double)]),
     "spgTune": (c_int, [c_void_p, c_void_p, c_int, c_uint64, c_char_p, POINTER(Config),
                         POINTER(c_double)]),
     "spgTuneLookup": (c_int, [c_void_p, c_char_p, c_int, POINTER(c_int)]),
@@ -513,12 +515,19 @@ class Context:
                 "spgBatchGetLayout")
         return ro, co
 
-    def batch_multiply(self, predictor=None, graph=False):
+    def batch_multiply(self, predictor=None, graph=False, latency=False):
         st = LearnedStats()
-        self._c(lib().spgBatchMultiply(self.h, predictor.h if predictor else None,
-                                       BATCH_GRAPH if graph else 0, byref(st)),
+        flags = (BATCH_GRAPH if graph else 0) | (BATCH_LATENCY if latency else 0)
+        self._c(lib().spgBatchMultiply(self.h, predictor.h if predictor else None, flags, byref(st)),
                 "spgBatchMultiply")
         return {f: getattr(st, f) for f, _ in LearnedStats._fields_} if predictor else None
+
+    def batch_latency(self, count):
+        """Per-product completion times (ms from the start of the run) of the
+        last batch_multiply(latency=True)."""
+        ms = np.zeros(count, np.float64)
+        self._c(lib().spgBatchGetLatency(self.h, _ptr(ms, c_double)), "spgBatchGetLatency")
+        return ms
 
     def batch_get_c(self, b):
         out = CSR64()

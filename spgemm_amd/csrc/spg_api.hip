@@ -230,7 +230,8 @@ static void for_each_buffer(spg_context* c, F f) {
                       &c->kitem_off, &c->inter, &c->bpair, &c->ebase, &c->vclass, &c->vlist,
                       &c->vclass_off, &c->vcnt, &c->vout, &c->crow, &c->ccol, &c->cval,
                       &c->pred, &c->vcap, &c->pbase, &c->fbase, &c->pcol, &c->pval,
-                      &c->fcol, &c->fval, &c->rwinb, &c->kwinb, &c->scratch};
+                      &c->fcol, &c->fval, &c->rwinb, &c->kwinb, &c->scratch,
+                      &c->lat_t, &c->rprod, &c->vprod};
     for (DevBuf* b : bufs) f(*b);
 }
 

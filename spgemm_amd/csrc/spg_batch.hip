@@ -50,7 +50,9 @@ int check_cm(spg_context* c, const cm& x, bool csc, int b) {
 }
 
 int run_batch(spg_context* c, const spg_predictor* p, spg_learned_stats* st) {
+    if (c->lat_on) SPG_TRY(launch_latency_start(c));
     SPG_TRY(spgInitGEMM(c, nullptr, nullptr));
+    if (c->lat_on) SPG_TRY(launch_vprod(c));
     SPG_TRY(spgCategorizeGEMM(c, nullptr));
     SPG_TRY(spgBinGEMM(c));
     if (p) {
@@ -94,6 +96,7 @@ int spgBatchSetOperands(spg_context* c, int count, const cm* A, const cm* B) {
     std::vector<int> ap((size_t)K + 1), ai((size_t)nnzA), bp((size_t)K + 1), bi((size_t)nnzB);
     std::vector<float> av((size_t)nnzA), bv((size_t)nnzB);
     std::vector<int2> rwin((size_t)std::max<int64_t>(m, 1)), kwin((size_t)std::max<int64_t>(K, 1));
+    std::vector<int32_t> rprod((size_t)std::max<int64_t>(m, 1));
     int64_t ea = 0, eb = 0;
     for (int b = 0; b < count; ++b) {
         const cm& a = A[b];
@@ -112,7 +115,10 @@ int spgBatchSetOperands(spg_context* c, int count, const cm* A, const cm* B) {
             bi[(size_t)(eb + e)] = x.idx[e] + (int)col_off[b];
             bv[(size_t)(eb + e)] = x.val[e];
         }
-        for (int i = 0; i < a.num_rows; ++i) rwin[(size_t)(row_off[b] + i)] = win;
+        for (int i = 0; i < a.num_rows; ++i) {
+            rwin[(size_t)(row_off[b] + i)] = win;
+            rprod[(size_t)(row_off[b] + i)] = b;
+        }
         ea += a.nnz;
         eb += x.nnz;
     }
@@ -125,6 +131,10 @@ int spgBatchSetOperands(spg_context* c, int count, const cm* A, const cm* B) {
     SPG_TRY(spgSetOperand(c, SPG_OPERAND_B, &bigB));
     SPG_TRY(ensure(c, c->rwinb, sizeof(int2) * rwin.size()));
     SPG_TRY(ensure(c, c->kwinb, sizeof(int2) * kwin.size()));
+    SPG_TRY(ensure(c, c->rprod, sizeof(int32_t) * rprod.size()));
+    SPG_TRY(ensure(c, c->lat_t, sizeof(uint64_t) * ((size_t)count + 1)));
+    SPG_HIP(hipMemcpyAsync(c->rprod.p, rprod.data(), sizeof(int32_t) * rprod.size(),
+                           hipMemcpyHostToDevice, c->stream));
     SPG_HIP(hipMemcpyAsync(c->rwinb.p, rwin.data(), sizeof(int2) * rwin.size(),
                            hipMemcpyHostToDevice, c->stream));
     SPG_HIP(hipMemcpyAsync(c->kwinb.p, kwin.data(), sizeof(int2) * kwin.size(),
@@ -153,21 +163,27 @@ int spgBatchGetLayout(spg_context* c, int64_t* row_off, int64_t* col_off) {
 int spgBatchMultiply(spg_context* c, const spg_predictor* p, int flags, spg_learned_stats* st) {
     if (!c) return SPG_EINVAL;
     if (c->batch_count <= 0) return fail(c, SPG_EORDER, "spgBatchSetOperands must run first");
-    if (flags & ~SPG_BATCH_GRAPH) return fail(c, SPG_EINVAL, "unknown batch flags");
+    if (flags & ~(SPG_BATCH_GRAPH | SPG_BATCH_LATENCY)) return fail(c, SPG_EINVAL, "unknown batch flags");
     SPG_HIP(hipSetDevice(c->device));
     spg_learned_stats local{};
     spg_learned_stats* sp = p ? &local : nullptr;
+    const bool lat = (flags & SPG_BATCH_LATENCY) != 0;
     if (!(flags & SPG_BATCH_GRAPH)) {
-        SPG_TRY(run_batch(c, p, sp));
+        c->lat_on = lat;
+        const int rc = run_batch(c, p, sp);
+        c->lat_on = false;
+        SPG_TRY(rc);
+        c->lat_valid = lat;
         if (st && p) *st = local;
         return SPG_OK;
     }
     if (!c->stream) return fail(c, SPG_EINVAL, "graph capture needs a non-default stream");
     const bool valid = c->graph_exec && c->graph_learned == (p != nullptr) && c->graph_pred == p &&
-                       (!p || predictor_version(p) == c->graph_pred_version);
+                       (!p || predictor_version(p) == c->graph_pred_version) && c->graph_lat == lat;
     if (valid) {
         SPG_HIP(hipGraphLaunch(c->graph_exec, c->stream));
         c->stage = 5;
+        c->lat_valid = lat;
         if (st && p) *st = c->graph_stats;  // the counts of the recorded run
         return SPG_OK;
     }
@@ -175,9 +191,13 @@ int spgBatchMultiply(spg_context* c, const spg_predictor* p, int flags, spg_lear
     // 1) an ordinary run whose host readbacks are recorded
     c->plan.clear();
     c->plan_mode = 1;
+    c->lat_on = lat;
     int rc = run_batch(c, p, sp);
     c->plan_mode = 0;
-    if (rc != SPG_OK) return rc;
+    if (rc != SPG_OK) {
+        c->lat_on = false;
+        return rc;
+    }
     if (p) c->graph_stats = local;
     // 2) the same sequence captured, with the readbacks replayed from the log
     SPG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
@@ -186,6 +206,7 @@ int spgBatchMultiply(spg_context* c, const spg_predictor* p, int flags, spg_lear
     spg_learned_stats scratch_stats{};
     rc = run_batch(c, p, p ? &scratch_stats : nullptr);
     c->plan_mode = 0;
+    c->lat_on = false;
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc != SPG_OK || e != hipSuccess || !g) {
@@ -204,8 +225,29 @@ int spgBatchMultiply(spg_context* c, const spg_predictor* p, int flags, spg_lear
     c->graph_learned = p != nullptr;
     c->graph_pred = p;
     c->graph_pred_version = p ? predictor_version(p) : -1;
+    c->graph_lat = lat;
+    c->lat_valid = lat;
     c->stage = 5;  // C of the recorded run is in place
     if (st && p) *st = local;
+    return SPG_OK;
+}
+
+int spgBatchGetLatency(spg_context* c, double* ms) {
+    if (!c || !ms) return SPG_EINVAL;
+    if (c->batch_count <= 0) return fail(c, SPG_EORDER, "spgBatchSetOperands must run first");
+    if (!c->lat_valid)
+        return fail(c, SPG_EORDER, "the last spgBatchMultiply did not record completion times");
+    SPG_HIP(hipSetDevice(c->device));
+    int khz = 0;
+    SPG_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+    if (khz <= 0) return fail(c, SPG_EHIP, "no wall-clock rate");
+    std::vector<uint64_t> t((size_t)c->batch_count + 1);
+    SPG_HIP(hipMemcpyAsync(t.data(), c->lat_t.p, sizeof(uint64_t) * t.size(), hipMemcpyDeviceToHost,
+                           c->stream));
+    SPG_HIP(hipStreamSynchronize(c->stream));
+    const uint64_t t0 = t[(size_t)c->batch_count];
+    for (int b = 0; b < c->batch_count; ++b)
+        ms[b] = t[(size_t)b] > t0 ? (double)(t[(size_t)b] - t0) / (double)khz : 0.0;
     return SPG_OK;
 }
 

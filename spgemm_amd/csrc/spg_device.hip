@@ -2074,6 +2074,18 @@ __device__ __forceinline__ int hash_insert(int* keys, int key, bool& fresh) {
     return -1;
 }
 
+// Completion times of batched products (spgBatchMultiply with
+// SPG_BATCH_LATENCY): the kernel that writes a virtual row's final C
+// entries raises its product's time to the wall clock once the row is
+// written.  t == null: nothing is recorded.
+struct Done {
+    unsigned long long* t;  // per product, wall-clock ticks
+    const int* vprod;       // product of every virtual row
+};
+__device__ __forceinline__ void mark_done(const Done& d, int v) {
+    if (d.t) atomicMax(&d.t[d.vprod[v]], (unsigned long long)wall_clock64());
+}
+
 // --------------------------------------------------------------- symbolic
 // Classes of a virtual row.  0..7: LDS hash with 2^(6+class) slots; TILE:
 // the row's column range fits one tile (bitmap / dense FP64 accumulator);
@@ -2307,7 +2319,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_hash_wave(
     const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
     const int* __restrict__ icol, const float* __restrict__ ival,
     const long long* __restrict__ vout, int* __restrict__ ccol, float* __restrict__ cval,
-    long long* __restrict__ cnt_out) {
+    long long* __restrict__ cnt_out, const Done dn) {
     constexpr int H = 1 << LOG2H, P = H / 2, SPL = H / 64;  // slots per lane
     static_assert(H >= 64 && H <= 512, "wave hash sizes");
     __shared__ int keysall[WPB][H];
@@ -2384,6 +2396,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_hash_wave(
             }
         }
         if (lane == 0 && cnt_out) cnt_out[v] = total;
+        if (lane == 0) mark_done(dn, v);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
     }
@@ -2513,7 +2526,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_hash(
     const int* __restrict__ vlist, const long long* __restrict__ vbase,
     const int* __restrict__ icol, const float* __restrict__ ival,
     const long long* __restrict__ vout, int* __restrict__ ccol, float* __restrict__ cval,
-    long long* __restrict__ cnt_out) {
+    long long* __restrict__ cnt_out, const Done dn) {
     constexpr int H = 1 << LOG2H;
     constexpr int P = H / 2;  // at most H/2 distinct keys are sorted
     __shared__ int keys[H];
@@ -2589,6 +2602,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_hash(
         ccol[o + q] = skey[q];
         cval[o + q] = (float)sval[q];
     }
+    if (threadIdx.x == 0) mark_done(dn, v);
 }
 
 // Tile-width virtual rows: dense FP64 accumulator over the tile's columns
@@ -2608,7 +2622,7 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
     const int* __restrict__ vlo, const int* __restrict__ vwid,
     const unsigned* __restrict__ tilebits, const int* __restrict__ icol,
     const float* __restrict__ ival, const long long* __restrict__ vout, int* __restrict__ ccol,
-    float* __restrict__ cval, long long* __restrict__ cnt_out) {
+    float* __restrict__ cval, long long* __restrict__ cnt_out, const Done dn) {
     constexpr int WMAX = 1 << kTileMaxLog2;
     constexpr int NW = WMAX / 32;
     static_assert(NW <= kPT, "one bitmap word per thread");
@@ -2725,6 +2739,7 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
             }
         }
         __syncthreads();
+        if (threadIdx.x == 0) mark_done(dn, v);
         if constexpr (BUILD) {  // cleared once every thread has emitted (the next row ORs into it)
             if (threadIdx.x < NW) bits[threadIdx.x] = 0u;
             __syncthreads();
@@ -2755,7 +2770,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
     const int* __restrict__ vlo, const unsigned* __restrict__ tilebits,
     const int* __restrict__ icol, const float* __restrict__ ival,
     const long long* __restrict__ vout, int* __restrict__ ccol, float* __restrict__ cval,
-    long long* __restrict__ cnt_out) {
+    long long* __restrict__ cnt_out, const Done dn) {
     __shared__ double accall[WPB][CAP];
     __shared__ int colall[WPB][CAP];
     __shared__ unsigned bitsall[WPB][kTileWords];
@@ -2863,6 +2878,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
             ccol[o0 + q] = colr[q];
             cval[o0 + q] = (float)acc[q];
         }
+        if (lane == 0) mark_done(dn, v);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
     }
@@ -2879,7 +2895,7 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
     const long long* __restrict__ vbase, const int* __restrict__ vlo,
     const int* __restrict__ icol, const float* __restrict__ ival,
     const long long* __restrict__ vout, int* __restrict__ ccol, float* __restrict__ cval,
-    long long* __restrict__ cnt_out) {
+    long long* __restrict__ cnt_out, const Done dn) {
     __shared__ unsigned bits[kRankMaxWords];
     __shared__ unsigned short pre[kRankMaxWords / 2];  // rank before each word pair
     __shared__ double acc[kHashMaxCount];
@@ -2998,6 +3014,7 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
                 }
             }
             __syncthreads();
+            if (threadIdx.x == 0) mark_done(dn, v);
         }
     }
 }
@@ -3056,7 +3073,7 @@ __global__ __launch_bounds__(64) void k_merge_rank_wave(
     const int* __restrict__ vlist, long long nlist, int G, const long long* __restrict__ vbase,
     const int* __restrict__ vlo, const int* __restrict__ vwid, const int* __restrict__ icol,
     const float* __restrict__ ival, const long long* __restrict__ vout, int* __restrict__ ccol,
-    float* __restrict__ cval, long long* __restrict__ cnt_out) {
+    float* __restrict__ cval, long long* __restrict__ cnt_out, const Done dn) {
     constexpr int PER = kRankWavePer;
     static_assert(NW % 2 == 0, "words are handled in pairs");
     __shared__ __align__(8) unsigned bits[NW];
@@ -3184,6 +3201,7 @@ __global__ __launch_bounds__(64) void k_merge_rank_wave(
                 ccol[o0 + q] = colr[q];
                 cval[o0 + q] = (float)acc[q];
             }
+            if (lane == 0) mark_done(dn, v);
             for (int w = w0; w < w1; ++w) bits[w] = 0u;  // a lane clears the words it counted
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -3199,13 +3217,24 @@ struct MergeOut {
     int* ocol;
     float* oval;
     long long* cnt;
+    Done dn;  // completion times of batched products (null: none)
 };
+
+// The context's completion-time recording (SPG_BATCH_LATENCY), or none.
+static Done done_of(const spg_context* c) {
+    Done d{nullptr, nullptr};
+    if (c->lat_on) {
+        d.t = (unsigned long long*)c->lat_t.p;
+        d.vprod = (const int*)c->vprod.p;
+    }
+    return d;
+}
 
 template <int LOG2H>
 static void launch_merge_hash(spg_context* c, const int* list, long long cnt, const MergeOut& o) {
     hipLaunchKernelGGL(k_merge_hash<LOG2H>, dim3((unsigned)cnt), dim3(kBlock), 0, c->stream,
                        list, (const long long*)c->vbase.p, (const int*)inter_cols(c),
-                       (const float*)inter_vals(c), o.obase, o.ocol, o.oval, o.cnt);
+                       (const float*)inter_vals(c), o.obase, o.ocol, o.oval, o.cnt, o.dn);
 }
 
 // One merge kernel over the virtual rows list[0..cnt) of class b.  build:
@@ -3221,31 +3250,31 @@ static int launch_merge_class(spg_context* c, int b, const int* list, long long 
         const dim3 g(persistent_grid(c, (cnt + 7) / 8, 4));
         if (build)
             hipLaunchKernelGGL((k_merge_tile_wave<256, 8, true>), g, dim3(512), 0, c->stream, list,
-                               cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval, o.cnt);
+                               cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
         else
             hipLaunchKernelGGL((k_merge_tile_wave<256, 8, false>), g, dim3(512), 0, c->stream, list,
-                               cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval, o.cnt);
+                               cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
     } else if (b == kClassTileM) {
         // one wave per workgroup: 13.8 KB of LDS each, so a CU holds 11 of
         // them (four-wave workgroups fit only twice: 8 waves)
         const dim3 g(persistent_grid(c, cnt, 11));
         if (build)
             hipLaunchKernelGGL((k_merge_tile_wave<1024, 1, true>), g, dim3(64), 0, c->stream, list,
-                               cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval, o.cnt);
+                               cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
         else
             hipLaunchKernelGGL((k_merge_tile_wave<1024, 1, false>), g, dim3(64), 0, c->stream,
                                list, cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval,
-                               o.cnt);
+                               o.cnt, o.dn);
     } else if (b == kClassTile) {
         const dim3 g(persistent_grid(c, cnt, 2));
         if (build)
             hipLaunchKernelGGL(k_merge_tile<true>, g, dim3(kPT), 0, c->stream, list, cnt, vbase,
                                vlo, (const int*)c->vwid.p, tb, icol, ival, o.obase, o.ocol,
-                               o.oval, o.cnt);
+                               o.oval, o.cnt, o.dn);
         else
             hipLaunchKernelGGL(k_merge_tile<false>, g, dim3(kPT), 0, c->stream, list, cnt, vbase,
                                vlo, (const int*)c->vwid.p, tb, icol, ival, o.obase, o.ocol,
-                               o.oval, o.cnt);
+                               o.oval, o.cnt, o.dn);
     } else if (b == kClassRankW) {
         // rows per wave: 64 when the list fills 7 waves per CU that many times
         const long long waves = (long long)c->num_cu * 7;
@@ -3253,34 +3282,34 @@ static int launch_merge_class(spg_context* c, int b, const int* list, long long 
         hipLaunchKernelGGL((k_merge_rank_wave<kRankWaveWords, kRankWaveCap>),
                            dim3(persistent_grid(c, (cnt + G - 1) / G, 7)), dim3(64), 0,
                            c->stream, list, cnt, G, vbase, vlo, (const int*)c->vwid.p, icol, ival,
-                           o.obase, o.ocol, o.oval, o.cnt);
+                           o.obase, o.ocol, o.oval, o.cnt, o.dn);
     } else if (b == kClassRank) {
         hipLaunchKernelGGL(k_merge_rank,
                            dim3(persistent_grid(c, (cnt + kRankChunk - 1) / kRankChunk, 2)),
                            dim3(kPT), 0, c->stream, list, cnt,
                            (unsigned long long*)c->counters.p + 2, vbase, vlo, icol, ival,
-                           o.obase, o.ocol, o.oval, o.cnt);
+                           o.obase, o.ocol, o.oval, o.cnt, o.dn);
     } else {
         const dim3 gw(persistent_grid(c, (cnt + kHashWaveWPB - 1) / kHashWaveWPB, 4));
         const dim3 bw(64 * kHashWaveWPB);
         switch (b + kHashMinLog2) {
             case 6:
                 hipLaunchKernelGGL((k_merge_hash_wave<6, kHashWaveWPB>), gw, bw, 0, c->stream,
-                                   list, cnt, vbase, icol, ival, o.obase, o.ocol, o.oval, o.cnt);
+                                   list, cnt, vbase, icol, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
                 break;
             case 7:
                 hipLaunchKernelGGL((k_merge_hash_wave<7, kHashWaveWPB>), gw, bw, 0, c->stream,
-                                   list, cnt, vbase, icol, ival, o.obase, o.ocol, o.oval, o.cnt);
+                                   list, cnt, vbase, icol, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
                 break;
             case 8:
                 hipLaunchKernelGGL((k_merge_hash_wave<8, kHashWaveWPB>), gw, bw, 0, c->stream,
-                                   list, cnt, vbase, icol, ival, o.obase, o.ocol, o.oval, o.cnt);
+                                   list, cnt, vbase, icol, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
                 break;
             case 9:
                 hipLaunchKernelGGL((k_merge_hash_wave<9, 4>),
                                    dim3(persistent_grid(c, (cnt + 3) / 4, 4)), dim3(256), 0,
                                    c->stream, list, cnt, vbase, icol, ival, o.obase, o.ocol,
-                                   o.oval, o.cnt);
+                                   o.oval, o.cnt, o.dn);
                 break;
             case 10: launch_merge_hash<10>(c, list, cnt, o); break;
             case 11: launch_merge_hash<11>(c, list, cnt, o); break;
@@ -3308,7 +3337,8 @@ int launch_merge(spg_context* c) {
     long long off[kNumVClasses + 1];
     SPG_TRY(class_offsets(c, off));
     const int* vl = (const int*)c->vlist.p;
-    const MergeOut o{(const long long*)c->vout.p, (int*)c->ccol.p, (float*)c->cval.p, nullptr};
+    const MergeOut o{(const long long*)c->vout.p, (int*)c->ccol.p, (float*)c->cval.p, nullptr,
+                     done_of(c)};
     tmark(c, SPG_PHASE_MERGE, 0);
     for (int b = kNumVClasses - 1; b >= 0; --b) {
         const long long cnt = off[b + 1] - off[b];
@@ -3484,7 +3514,7 @@ __global__ __launch_bounds__(kPT) void k_learned_compact(
     const long long* __restrict__ pbase, const long long* __restrict__ fbase,
     const int* __restrict__ pcol, const float* __restrict__ pval, const int* __restrict__ fcol,
     const float* __restrict__ fval, int* __restrict__ ccol, float* __restrict__ cval,
-    unsigned long long* __restrict__ unrecovered) {
+    unsigned long long* __restrict__ unrecovered, const Done dn) {
     const int wv = threadIdx.x >> 6, lane = lane_id();
     constexpr int NWV = kPT / 64;
     const long long ngroups = (V + 63) / 64;
@@ -3570,6 +3600,12 @@ __global__ __launch_bounds__(kPT) void k_learned_compact(
                     }
                 }
             }
+        if (dn.t) {  // uniform: this group's rows are written; one atomic per product
+            const int pv = v < V && cnt > 0 ? dn.vprod[v] : -1;
+            const int pp = __shfl_up(pv, 1, 64);
+            if (pv >= 0 && (lane == 0 || pp != pv))
+                atomicMax(&dn.t[pv], (unsigned long long)wall_clock64());
+        }
             continue;
         }
         for (int pb = 0; pb < total; pb += 4 * 64) {
@@ -3634,6 +3670,12 @@ __global__ __launch_bounds__(kPT) void k_learned_compact(
                 }
             }
         }
+        if (dn.t) {  // uniform: this group's rows are written; one atomic per product
+            const int pv = v < V && cnt > 0 ? dn.vprod[v] : -1;
+            const int pp = __shfl_up(pv, 1, 64);
+            if (pv >= 0 && (lane == 0 || pp != pv))
+                atomicMax(&dn.t[pv], (unsigned long long)wall_clock64());
+        }
     }
 }
 
@@ -3697,7 +3739,7 @@ int launch_learned_merge(spg_context* c, spg_learned_stats* st) {
     SPG_TRY(class_offsets(c, off));
     const int* vl = (const int*)c->vlist.p;
     {
-        const MergeOut o{pbase, (int*)c->pcol.p, (float*)c->pval.p, vcnt};
+        const MergeOut o{pbase, (int*)c->pcol.p, (float*)c->pval.p, vcnt, Done{nullptr, nullptr}};
         for (int b = kNumVClasses - 1; b >= 0; --b) {
             const long long cnt = off[b + 1] - off[b];
             if (cnt > 0) SPG_TRY(launch_merge_class(c, b, vl + off[b], cnt, o, true));
@@ -3723,7 +3765,7 @@ int launch_learned_merge(spg_context* c, spg_learned_stats* st) {
                                  (int32_t*)c->vlist.p, (int64_t*)c->vclass_off.p));
         SPG_TRY(class_offsets(c, off));
         SPG_TRY(fill<unsigned long long>(c, ctr + kCtrRank, 0ull, 1));
-        const MergeOut o{fbase, (int*)c->fcol.p, (float*)c->fval.p, vcnt};
+        const MergeOut o{fbase, (int*)c->fcol.p, (float*)c->fval.p, vcnt, Done{nullptr, nullptr}};
         for (int b = kNumVClasses - 1; b >= 0; --b) {
             const long long cnt = off[b + 1] - off[b];
             if (cnt > 0) SPG_TRY(launch_merge_class(c, b, vl + off[b], cnt, o, true));
@@ -3747,7 +3789,7 @@ int launch_learned_merge(spg_context* c, spg_learned_stats* st) {
                            0, c->stream, V, (const long long*)vcnt, (const long long*)c->vout.p,
                            (const long long*)pbase, (const long long*)fbase,
                            (const int*)c->pcol.p, (const float*)c->pval.p, fcol, fval,
-                           (int*)c->ccol.p, (float*)c->cval.p, ctr + kCtrUnrec);
+                           (int*)c->ccol.p, (float*)c->cval.p, ctr + kCtrUnrec, done_of(c));
         SPG_HIP(hipGetLastError());
     }
     tmark(c, SPG_PHASE_COMPACT, 1);
@@ -3760,6 +3802,36 @@ int launch_learned_merge(spg_context* c, spg_learned_stats* st) {
         st->padded_slots = padded;
         st->fallback_slots = fslots;
         st->nnz = c->nnzC;
+    }
+    return SPG_OK;
+}
+
+// ------------------------------------------------ batch completion times
+__global__ void k_vprod(long long V, const int* __restrict__ vrow_row,
+                        const int* __restrict__ rprod, int* __restrict__ vprod) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) vprod[v] = rprod[vrow_row[v]];
+}
+
+__global__ void k_stamp(unsigned long long* t) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *t = (unsigned long long)wall_clock64();
+}
+
+int launch_latency_start(spg_context* c) {
+    unsigned long long* t = (unsigned long long*)c->lat_t.p;
+    SPG_TRY(fill<unsigned long long>(c, t, 0ull, c->batch_count));
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, c->stream, t + c->batch_count);
+    SPG_HIP(hipGetLastError());
+    return SPG_OK;
+}
+
+int launch_vprod(spg_context* c) {
+    SPG_TRY(ensure(c, c->vprod, sizeof(int) * ((size_t)c->V + 1)));
+    if (c->V > 0) {
+        hipLaunchKernelGGL(k_vprod, dim3((unsigned)((c->V + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           c->stream, (long long)c->V, (const int*)c->vrow_row.p,
+                           (const int*)c->rprod.p, (int*)c->vprod.p);
+        SPG_HIP(hipGetLastError());
     }
     return SPG_OK;
 }

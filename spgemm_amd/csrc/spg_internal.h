@@ -133,7 +133,14 @@ struct spg_context {
     int batch_count = 0;
     std::vector<int64_t> b_row_off, b_k_off, b_col_off;
     std::vector<int32_t> b_ncols;
+    // completion time of every product (SPG_BATCH_LATENCY): lat_t holds
+    // batch_count wall-clock times and then the run's start time; rprod is
+    // the product of every stacked row, vprod of every virtual row
+    bool lat_on = false;     // the run in progress records them
+    bool lat_valid = false;  // the last batch run recorded them
+    spg::DevBuf lat_t, rprod, vprod;
     // captured batch run
+    bool graph_lat = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool graph_learned = false;
@@ -215,6 +222,10 @@ int predictor_run(spg_context* c, const spg_predictor* p, float scale, float* d_
 int predictor_device(const spg_predictor* p);
 long long predictor_version(const spg_predictor* p);
 int launch_regroup_intermediate(spg_context* c, int64_t* d_rowptr);
+// Batch completion times: clears the per-product times and stamps the
+// start; the product of every virtual row (after InitGEMM).
+int launch_latency_start(spg_context* c);
+int launch_vprod(spg_context* c);
 
 }  // namespace spg
 

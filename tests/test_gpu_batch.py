@@ -166,3 +166,45 @@ def test_batch_then_single_product(spg, orc, gpu_ctx):
     check(ctx.get_c(), oracle_c(orc, R, R))
     with pytest.raises(spg.SpgError):
         ctx.batch_multiply()
+
+
+@pytest.mark.parametrize("learned", [False, True], ids=["plain", "learned"])
+def test_batch_completion_times(spg, orc, gpu_ctx, learned):
+    """SPG_BATCH_LATENCY: every product's completion time from the device
+    wall clock, eager and in a replayed graph.  The times lie between 0 and
+    the host-timed length of the run, a product without entries reports 0,
+    C is the same as without recording, and a later run without the flag
+    leaves no times to read."""
+    import time
+    from spgemm_amd import train_predictor as tp
+    print(f"case batch completion times learned={learned}", flush=True)
+    ctx = gpu_ctx
+    pairs = make_batch(spg, 71)
+    ctx.batch_set_operands(pairs)
+    P = tp.load_or_train(ctx)[0] if learned else None
+    try:
+        ctx.batch_multiply(predictor=P)
+        ref = [ctx.batch_get_c(b) for b in range(len(pairs))]
+        for graph in (False, True, True):  # eager, record + capture, replay
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            ctx.batch_multiply(predictor=P, graph=graph, latency=True)
+            ctx.synchronize()
+            wall = (time.perf_counter() - t0) * 1e3
+            ms = ctx.batch_latency(len(pairs))
+            nnz = np.array([ref[b][0][-1] for b in range(len(pairs))])
+            assert np.all(ms[nnz > 0] > 0) and np.all(ms[nnz == 0] == 0)
+            assert ms.max() <= wall * 1.05 + 0.05, (ms.max(), wall)
+            for b in range(len(pairs)):
+                C = ctx.batch_get_c(b)
+                assert np.array_equal(C[0], ref[b][0]) and np.array_equal(C[1], ref[b][1])
+                assert np.array_equal(C[2], ref[b][2])
+            print(f"  graph={graph}: p50 {np.percentile(ms[nnz > 0], 50):.3f} ms, "
+                  f"max {ms.max():.3f} ms, run {wall:.3f} ms")
+        ctx.batch_multiply(predictor=P, graph=True)  # no recording: a new graph
+        ctx.synchronize()
+        with pytest.raises(spg.SpgError):
+            ctx.batch_latency(len(pairs))
+    finally:
+        if P is not None:
+            P.close()
