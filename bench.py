@@ -391,19 +391,23 @@ def batch_result(args, pairs, workload):
 
 
 def bench_batch_only(args, pairs):
-    """The batched runs alone (for profiling): eager plain and learned
-    passes, then graph replays of the learned pass, with the learned pass's
-    settings (--tune-table entry, else tuned here first)."""
+    """The batched runs alone (for profiling): warmup + steps eager passes
+    of the --path paths (plain, learned or both), then, unless --no-graph,
+    steps graph calls of the learned pass (the first records and captures),
+    with the learned pass's settings (--set, --tune-table entry, else tuned
+    here first)."""
     from spgemm_amd import train_predictor as tp
     ctx = spg.Context(0)
     P, _ = tp.load_or_train(ctx)
     ctx.batch_set_operands(pairs)
     apply_tuning(ctx, args, P, True)
+    preds = {"both": (None, P), "learned": (P,), "plain": (None,)}[args.path]
     for _ in range(args.warmup + args.steps):
-        ctx.batch_multiply()
-        ctx.batch_multiply(predictor=P)
-    for _ in range(args.steps):
-        ctx.batch_multiply(predictor=P, graph=True)
+        for pred in preds:
+            ctx.batch_multiply(predictor=pred)
+    if not args.no_graph and args.path != "plain":
+        for _ in range(args.steps):
+            ctx.batch_multiply(predictor=P, graph=True)
     ctx.synchronize()
     print(json.dumps({"batch_only": True, "phase_ms": ctx.phase_times()}), flush=True)
     P.close()
@@ -523,6 +527,8 @@ def main():
                     help="JSON-lines table: apply its entry for the workload, else tune and append")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (counter calibration)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="--batch-only: eager passes only (no graph replays)")
     ap.add_argument("--batch-only", action="store_true",
                     help="batch workloads: skip the one-product-per-launch baseline (profiling)")
     ap.add_argument("--mtx-list", default=None,

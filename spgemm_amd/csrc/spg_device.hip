@@ -927,13 +927,22 @@ __global__ __launch_bounds__(kBlock) void k_limits(long long V, const long long*
     }
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < K; k += stride)
         mb = max(mb, (long long)(b_ptr[k + 1] - b_ptr[k]));
+    // one atomic per workgroup and value (atomics of a whole grid on one
+    // address serialize)
+    __shared__ long long part[3][kBlock / 64];
     mv = wave_max(mv);
     mu = wave_max(mu);
     mb = wave_max(mb);
     if (lane_id() == 0) {
-        if (mv) atomicMax(&out[0], (unsigned long long)mv);
-        if (mu) atomicMax(&out[1], (unsigned long long)mu);
-        if (mb) atomicMax(&out[2], (unsigned long long)mb);
+        part[0][threadIdx.x >> 6] = mv;
+        part[1][threadIdx.x >> 6] = mu;
+        part[2][threadIdx.x >> 6] = mb;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        long long x = 0;
+        for (int w = 0; w < kBlock / 64; ++w) x = max(x, part[threadIdx.x][w]);
+        if (x) atomicMax(&out[threadIdx.x], (unsigned long long)x);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) out[3] = (unsigned long long)vbase[V];
 }
@@ -941,7 +950,7 @@ __global__ __launch_bounds__(kBlock) void k_limits(long long V, const long long*
 int launch_limits(spg_context* c, unsigned long long* d_out) {
     SPG_TRY(fill<unsigned long long>(c, d_out, 0ull, 4));
     const long long work = std::max<long long>(c->V, c->K);
-    const unsigned g = persistent_grid(c, (work + kBlock - 1) / kBlock, 4);
+    const unsigned g = persistent_grid(c, (work + kBlock - 1) / kBlock, 1);
     hipLaunchKernelGGL(k_limits, dim3(g), dim3(kBlock), 0, c->stream, c->V,
                        (const long long*)c->vub.p, (const int*)c->vrow_row.p,
                        (const int*)c->vstart.p, c->K, c->B.ptr, (const long long*)c->vbase.p,
