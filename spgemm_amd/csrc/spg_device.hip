@@ -33,6 +33,14 @@ namespace spg {
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
+// A value every lane of the wave holds, moved to a scalar register.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ long long uni64(long long x) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)x);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(x >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
 __device__ __forceinline__ unsigned long long lanemask_lt() {
     return (1ull << lane_id()) - 1ull;
 }
@@ -1570,7 +1578,12 @@ constexpr int kRCMaxTiles = 256;  // rows with more tiles: per-vrow kernels
 constexpr int kRCMaxE = 64;       // entries per chunk
 constexpr int kRCU = 8;           // (tile, block) pairs in flight per wave
 
-__device__ __forceinline__ int rc_chunk_entries(int Ti) { return min(kRCMaxE, kRCTab / Ti); }
+// Entries per chunk: the largest power of two <= min(64, kRCTab / Ti), so
+// that a tile's row of the chunk table is searched by binary lifting.
+__device__ __forceinline__ int rc_chunk_entries(int Ti) {
+    const int e = min(kRCMaxE, kRCTab / Ti);
+    return 1 << (31 - __clz(e));
+}
 
 struct RCArgs {
     const int* rk;
@@ -1609,59 +1622,60 @@ __global__ void k_rc_units(const int* __restrict__ rows, long long nrows,
     for (int c0 = e0; c0 < e1; c0 += E) units[u++] = make_int4(vs, Ti, c0, min(E, e1 - c0));
 }
 
+template <bool B32>
 __global__ __launch_bounds__(kBlock) void k_expand_rowchunks(RCArgs g, const int4* __restrict__ units,
                                                               int nunits,
                                                               unsigned long long* __restrict__ ctr) {
-    __shared__ int pref[kRCTab + kRCMaxTiles];  // [e][t]: tile t's run offset of entry e (e <= ne)
-    __shared__ int xs[kRCTab];                  // [e][t]: B position of the piece's start
-    __shared__ long long sbase[kRCMaxTiles];
+    // [t][e], row stride E + 1: tile t's run offset of entry e (INT_MAX for
+    // e >= ne, so a row is searched with no bound test) and the B position
+    // of the entry's piece in the tile
+    __shared__ int2 tab[kRCTab + kRCMaxTiles];
+    __shared__ long long sbase[kRCMaxTiles];  // first slot of the chunk's run in tile t
+    __shared__ int rlen[kRCMaxTiles];         // length of that run
     __shared__ int gsum[kBlock];
-    __shared__ int tpb[kRCMaxTiles + 1];        // 64-slot blocks of the tiles, scanned
-    __shared__ int sk[kRCMaxE];
+    __shared__ int tpb[kRCMaxTiles + 1];      // 64-slot blocks of the tiles, scanned
     __shared__ float sa[kRCMaxE];
     __shared__ int sh[kBlock / 64];
     __shared__ int snext;
     constexpr int NW = kBlock / 64;
-    const int wv = threadIdx.x >> 6, lane = lane_id();
-    const int2* __restrict__ bpair = g.bpair;
-    int* __restrict__ icol = g.icol;
-    float* __restrict__ ival = g.ival;
+    const int wv = uni(threadIdx.x >> 6), lane = lane_id();
+    const char* __restrict__ bpair = (const char*)g.bpair;
     // the next unit is taken from the queue (and its description loaded)
     // while the current one is expanded
     if (threadIdx.x == 0) snext = (int)atomicAdd(ctr, 1ull);
     __syncthreads();
-    int u = snext;
+    int u = uni(snext);
     int4 inf = u < nunits ? units[u] : make_int4(0, 1, 0, 0);
     while (u < nunits) {
         unsigned long long nx = 0;
         if (threadIdx.x == 0) nx = atomicAdd(ctr, 1ull);
-        const int vs = inf.x, Ti = inf.y, c0 = inf.z, ne = inf.w;  // Ti in 2 .. kRCMaxTiles
-        const int E = rc_chunk_entries(Ti);
+        // Ti in 2 .. kRCMaxTiles
+        const int vs = uni(inf.x), Ti = uni(inf.y), c0 = uni(inf.z), ne = uni(inf.w);
+        const int E = rc_chunk_entries(Ti), RS = E + 1;
         const int NG = kBlock / Ti;         // entry groups of the prefix pass
-        const int EPG = (E + NG - 1) / NG;  // entries per group
+        const int EPG = (E + NG - 1) / NG;  // entries per group: <= 8, as E * Ti <= kRCTab
         const int t = threadIdx.x % Ti, eg = threadIdx.x / Ti;
-        if (threadIdx.x < ne) {
-            sk[threadIdx.x] = g.rk[c0 + threadIdx.x];
-            sa[threadIdx.x] = g.rv[c0 + threadIdx.x];
-        }
+        const int ea = eg * EPG;
+        if (threadIdx.x < ne) sa[threadIdx.x] = g.rv[c0 + threadIdx.x];
         if (threadIdx.x == 0) snext = (int)nx;
-        __syncthreads();
-        const int un = snext;
-        const int4 infn = un < nunits ? units[un] : make_int4(0, 1, 0, 0);
-        const int ea = eg * EPG, eb = min(ea + EPG, ne);
-        if (eg < NG) {  // tile t's piece of every entry in [ea, eb): start and run offset
-            int run = 0;
-#pragma unroll 4
-            for (int e = ea; e < eb; ++e) {
-                const int* bs = g.bsplit + (long long)sk[e] * (g.T + 1) + t;
-                const int x = bs[0], y = bs[1];
-                xs[e * Ti + t] = x;
-                pref[e * Ti + t] = run;
-                run += y - x;
+        // tile t's piece of every entry in [ea, ea + EPG): B start and length
+        int xv[8], lv[8], run = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int e = ea + j;
+            xv[j] = 0;
+            lv[j] = 0;
+            if (eg < NG && j < EPG && e < ne) {
+                const int* bs = g.bsplit + (long long)g.rk[c0 + e] * (g.T + 1) + t;
+                xv[j] = bs[0];
+                lv[j] = bs[1] - xv[j];
             }
-            gsum[eg * Ti + t] = run;
+            run += lv[j];
         }
+        if (eg < NG) gsum[eg * Ti + t] = run;
         __syncthreads();
+        const int un = uni(snext);
+        const int4 infn = un < nunits ? units[un] : make_int4(0, 1, 0, 0);
         int nbt = 0;
         if (threadIdx.x < Ti) {  // the chunk's run of every tile, reserved
             int sum = 0;
@@ -1669,56 +1683,92 @@ __global__ __launch_bounds__(kBlock) void k_expand_rowchunks(RCArgs g, const int
             long long base = 0;
             if (sum > 0) base = g.vbase[vs + threadIdx.x] + atomicAdd(&g.cursor[vs + threadIdx.x], sum);
             sbase[threadIdx.x] = base;
-            pref[ne * Ti + threadIdx.x] = sum;
+            rlen[threadIdx.x] = sum;
             nbt = (sum + 63) >> 6;
+        }
+        if (eg < NG) {  // the entries' run offsets, INT_MAX past the chunk's last entry
+            int pre = 0;
+            for (int q = 0; q < eg; ++q) pre += gsum[q * Ti + t];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int e = ea + j;
+                if (j < EPG && e < E) tab[t * RS + e] = make_int2(e < ne ? pre : INT_MAX, xv[j]);
+                pre += lv[j];
+            }
         }
         int nbk;
         const int tb = block_exscan<int>(nbt, &nbk, sh);
         if (threadIdx.x < Ti) tpb[threadIdx.x] = tb;
         if (threadIdx.x == 0) tpb[Ti] = nbk;
-        if (eg < NG) {
-            int pre = 0;
-            for (int q = 0; q < eg; ++q) pre += gsum[q * Ti + t];
-            for (int e = ea; e < eb; ++e) pref[e * Ti + t] += pre;
-        }
         __syncthreads();
         // (tile, 64-slot block) pairs, a contiguous share per wave: the lanes
-        // of one store cover 64 consecutive slots of one tile's run
-        const int per = (nbk + NW - 1) / NW;
-        const int pb = min(nbk, wv * per), pe = min(nbk, pb + per);
-        int tc = 0;
-        for (int p0 = pb; p0 < pe; p0 += kRCU) {
-            int tv[kRCU], pos[kRCU], ev[kRCU];
-#pragma unroll
-            for (int v = 0; v < kRCU; ++v) {
-                const int P = min(p0 + v, pe - 1);
-                while (tpb[tc + 1] <= P) ++tc;
-                tv[v] = tc;
-                pos[v] = min(((P - tpb[tc]) << 6) + lane, pref[ne * Ti + tc] - 1);
-                ev[v] = 0;  // largest entry whose run offset is <= pos
+        // of one store cover 64 consecutive slots of one tile's run.  A
+        // block's tile, run and first slot are wave-uniform (scalar); each
+        // lane finds its entry by binary lifting over the tile's row.
+        const int nb = uni(nbk);
+        const int per = (nb + NW - 1) / NW;
+        const int pb = min(nb, wv * per), pe = min(nb, pb + per);
+        if (pb < pe) {
+            int cnt = 0;  // the tile of block pb: the last t with tpb[t] <= pb
+            for (int t0 = 0; t0 < Ti; t0 += 64) {
+                const int tt = t0 + lane;
+                cnt += __popcll(__ballot(tt < Ti && tpb[tt] <= pb));
             }
-#pragma unroll
-            for (int step = kRCMaxE / 2; step > 0; step >>= 1) {
+            int tc = cnt - 1;
+            int t_beg = uni(tpb[tc]), t_end = uni(tpb[tc + 1]), rl = uni(rlen[tc]);
+            long long sb = uni64(sbase[tc]);
+            for (int p0 = pb; p0 < pe; p0 += kRCU) {
+                int row[kRCU], boff[kRCU], pos[kRCU];
+                unsigned ra[kRCU];  // byte offset in tab of the entry found so far
+                long long s0[kRCU];
 #pragma unroll
                 for (int v = 0; v < kRCU; ++v) {
-                    const int cand = ev[v] + step;
-                    if (cand < ne && pref[cand * Ti + tv[v]] <= pos[v]) ev[v] = cand;
+                    const int P = min(p0 + v, pe - 1);
+                    while (P >= t_end) {
+                        ++tc;
+                        t_beg = t_end;
+                        t_end = uni(tpb[tc + 1]);
+                        rl = uni(rlen[tc]);
+                        sb = uni64(sbase[tc]);
+                    }
+                    row[v] = tc * RS;
+                    boff[v] = (P - t_beg) << 6;
+                    s0[v] = sb + boff[v];
+                    pos[v] = min(boff[v] + lane, rl - 1);
+                    ra[v] = (unsigned)row[v] * sizeof(int2);  // entry 0: run offset 0 <= pos
                 }
-            }
-            int col[kRCU];
-            float bv[kRCU];
+                // the largest entry whose run offset is <= pos
 #pragma unroll
-            for (int v = 0; v < kRCU; ++v) {
-                const int q = xs[ev[v] * Ti + tv[v]] + pos[v] - pref[ev[v] * Ti + tv[v]];
-                const int2 pr = bpair[q];
-                col[v] = pr.x;
-                bv[v] = __int_as_float(pr.y);
-            }
+                for (int step = kRCMaxE / 2; step > 0; step >>= 1) {
+                    if (step < E) {
 #pragma unroll
-            for (int v = 0; v < kRCU; ++v) {
-                const long long w = islot(sbase[tv[v]] + pos[v]);
-                icol[w] = col[v];
-                ival[w] = __fmul_rn(sa[ev[v]], bv[v]);
+                        for (int v = 0; v < kRCU; ++v) {
+                            const unsigned cand = ra[v] + step * (unsigned)sizeof(int2);
+                            const int x = *(const int*)((const char*)tab + cand);
+                            ra[v] = x <= pos[v] ? cand : ra[v];
+                        }
+                    }
+                }
+                int col[kRCU];
+                float val[kRCU];
+#pragma unroll
+                for (int v = 0; v < kRCU; ++v) {
+                    const int2 pr = *(const int2*)((const char*)tab + ra[v]);
+                    const int ev = (int)(ra[v] / sizeof(int2)) - row[v];
+                    const unsigned q = (unsigned)(pr.y + pos[v] - pr.x);
+                    const int2 b = B32 ? *(const int2*)(bpair + (q << 3)) : ((const int2*)bpair)[q];
+                    col[v] = b.x;
+                    val[v] = __fmul_rn(sa[ev], __int_as_float(b.y));
+                }
+#pragma unroll
+                for (int v = 0; v < kRCU; ++v) {
+                    // slot s0 + d sits at word (s0 >> 6) * 128 + o + (o & 64), o = (s0 & 63) + d
+                    const long long w0 = (s0[v] >> kIGroupLog2) << (kIGroupLog2 + 1);
+                    const int o = (int)(s0[v] & (kIGroup - 1)) + (pos[v] - boff[v]);
+                    const unsigned off = (unsigned)(o + (o & kIGroup)) << 2;
+                    *(int*)((char*)(g.icol + w0) + off) = col[v];
+                    *(float*)((char*)(g.ival + w0) + off) = val[v];
+                }
             }
         }
         __syncthreads();  // the unit's LDS tables are free
@@ -1941,8 +1991,14 @@ static int launch_rowchunks(spg_context* c, const RCPlan& pl) {
     h.T = c->T;
     h.icol = inter_cols(c);
     h.ival = inter_vals(c);
-    hipLaunchKernelGGL(k_expand_rowchunks, dim3(persistent_grid(c, pl.nunits, 6)), dim3(kBlock), 0,
-                       c->stream, h, pl.units, pl.nunits, ctr);
+    // B's pairs addressed by 32-bit byte offsets while they fit in 4 GiB
+    const dim3 grid(persistent_grid(c, pl.nunits, 6));
+    if (c->B.nnz <= (1LL << 29))
+        hipLaunchKernelGGL(k_expand_rowchunks<true>, grid, dim3(kBlock), 0, c->stream, h, pl.units,
+                           pl.nunits, ctr);
+    else
+        hipLaunchKernelGGL(k_expand_rowchunks<false>, grid, dim3(kBlock), 0, c->stream, h, pl.units,
+                           pl.nunits, ctr);
     SPG_HIP(hipGetLastError());
     return SPG_OK;
 }
@@ -2619,12 +2675,13 @@ __global__ __launch_bounds__(kBlock) void k_merge_hash(
 // BUILD when none ran).  Persistent workgroups walk the list with a fixed
 // stride; each thread loads two groups of four consecutive products per
 // pass (16-byte loads) and the next pass's groups are in flight while the
-// current ones are accumulated; the next row's descriptor and first products are loaded
-// before the current row is emitted.  The emit walks the columns in order,
-// one per thread (consecutive threads read consecutive accumulators, so no
-// LDS bank is hit twice, and write consecutive output positions); its rank
-// is the word's prefix plus the bits below it.  Only touched accumulators
-// are cleared (while being emitted).
+// current ones are accumulated.  A row's descriptor is loaded one row ahead
+// and its first products while the row before it is emitted.  The emit
+// gives each wave 32 consecutive bitmap words (1024 columns) held in its
+// lanes: 64 columns at a time, the two words form a scalar mask, a column's
+// rank is the wave's base plus the mask bits below its lane, and the set
+// columns are written to consecutive output positions.  Only touched
+// accumulators are cleared (while being emitted).
 template <bool BUILD>
 __global__ __launch_bounds__(kPT) void k_merge_tile(
     const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
@@ -2634,20 +2691,34 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
     float* __restrict__ cval, long long* __restrict__ cnt_out, const Done dn) {
     constexpr int WMAX = 1 << kTileMaxLog2;
     constexpr int NW = WMAX / 32;
+    constexpr int NWV = kPT / 64;
     static_assert(NW <= kPT, "one bitmap word per thread");
+    static_assert(NW == 32 * NWV, "32 bitmap words per wave");
     __shared__ double acc[WMAX];
     __shared__ unsigned bits[NW];
-    __shared__ unsigned short pre[NW];  // rank before each word
-    __shared__ int sh[kPT / 64];
+    __shared__ int wtot[NWV];  // set columns of each wave's words
     for (int x = threadIdx.x; x < WMAX; x += kPT) acc[x] = 0.0;
     for (int w = threadIdx.x; w < NW; w += kPT) bits[w] = 0u;
     __syncthreads();
     long long j = blockIdx.x;
     if (j >= nlist) return;
     const long long G = gridDim.x;
+    const int wv = threadIdx.x >> 6, lane = lane_id();
     int v = vlist[j];
     long long s0 = vbase[v], s1 = vbase[v + 1], o0 = vout[v], lim = vout[v + 1];
     int lo = vlo[v], nw = (vwid[v] + 31) >> 5;
+    // the next row's descriptor, one row ahead
+    int vn = j + G < nlist ? vlist[j + G] : -1;
+    long long s0n = 0, s1n = 0, o0n = 0, limn = 0;
+    int lon = 0, nwn = 0;
+    if (vn >= 0) {
+        s0n = vbase[vn];
+        s1n = vbase[vn + 1];
+        o0n = vout[vn];
+        limn = vout[vn + 1];
+        lon = vlo[vn];
+        nwn = (vwid[vn] + 31) >> 5;
+    }
     // rows of the class are non-empty; a 16-byte group past the row's end
     // is clamped to the group holding its last product (the intermediates
     // are padded to whole groups), and its lanes are masked where used
@@ -2664,8 +2735,8 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
         qv[d] = *(const float4*)(ival + islot(b0));
     }
     for (;;) {
-        const long long jn = j + G;
-        const int vn = jn < nlist ? vlist[jn] : -1;
+        const long long jnn = j + 2 * G;
+        const int vnn = vn >= 0 && jnn < nlist ? vlist[jnn] : -1;
         unsigned wb = 0u;  // this row's bitmap word (exact path)
         if (!BUILD && threadIdx.x < nw) wb = tilebits[(long long)v * kTileWords + threadIdx.x];
         for (long long sb = s0 & ~3LL;;) {
@@ -2711,16 +2782,8 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
         if constexpr (!BUILD) {
             if (threadIdx.x < NW) bits[threadIdx.x] = wb;  // zero past the window
         }
-        __syncthreads();
-        long long s0n = 0, s1n = 0, o0n = 0, limn = 0;
-        int lon = 0, nwn = 0;
-        if (vn >= 0) {  // next row's descriptor and first products, before the emit
-            s0n = vbase[vn];
-            s1n = vbase[vn + 1];
-            o0n = vout[vn];
-            limn = vout[vn + 1];
-            lon = vlo[vn];
-            nwn = (vwid[vn] + 31) >> 5;
+        __syncthreads();  // the row's accumulators and bitmap are complete
+        if (vn >= 0) {  // the next row's first products, in flight during the emit
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 const long long nb = min((s0n & ~3LL) + (long long)(d * kPT + threadIdx.x) * 4,
@@ -2729,32 +2792,50 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
                 qv[d] = *(const float4*)(ival + islot(nb));
             }
         }
-        const unsigned b = threadIdx.x < NW ? bits[threadIdx.x] : 0u;
-        int tot;
-        const int pw = block_exscan_n<int, kPT>(__popc(b), &tot, sh);
-        if (threadIdx.x < NW) pre[threadIdx.x] = (unsigned short)pw;
-        if (cnt_out && threadIdx.x == 0) cnt_out[v] = tot;
+        // the wave's 32 words, one per lane (lanes 32..63 hold zero)
+        const unsigned bw = lane < 32 ? bits[wv * 32 + lane] : 0u;
+        if constexpr (BUILD) {
+            if (lane < 32) bits[wv * 32 + lane] = 0u;  // read only by this wave; the next row ORs into it
+        }
+        int wc = __popc(bw);
+#pragma unroll
+        for (int o = 16; o > 0; o >>= 1) wc += __shfl_xor(wc, o, 64);
+        if (lane == 0) wtot[wv] = wc;
         __syncthreads();
-        for (int col = threadIdx.x; col < nw * 32; col += kPT) {
-            const int w = col >> 5;
-            const unsigned bw = bits[w];
-            if ((bw >> (col & 31)) & 1u) {
-                const long long o = o0 + pre[w] + __popc(bw & ((1u << (col & 31)) - 1u));
+        long long ob = o0;
+        int tot = 0;
+#pragma unroll
+        for (int w = 0; w < NWV; ++w) {
+            const int x = wtot[w];
+            if (w < wv) ob += x;
+            tot += x;
+        }
+        if (cnt_out && threadIdx.x == 0) cnt_out[v] = tot;
+        ob = uni64(ob);
+        const int ncol = uni(nw) * 32, wcol = wv * 1024;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            if (wcol + 64 * i >= ncol) break;  // past the row's window
+            const unsigned long long m =
+                ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)bw, 2 * i + 1) << 32) |
+                (unsigned)__builtin_amdgcn_readlane((int)bw, 2 * i);
+            if (m == 0ull) continue;
+            if ((m >> lane) & 1ull) {
+                const int col = wcol + 64 * i + lane;
+                const long long o = ob + __popcll(m & lanemask_lt());
+                const double a = acc[col];
+                acc[col] = 0.0;
                 if (o < lim) {  // slot range fixed by the symbolic pass
                     ccol[o] = lo + col;
-                    cval[o] = (float)acc[col];
+                    cval[o] = (float)a;
                 }
-                acc[col] = 0.0;
             }
+            ob += __popcll(m);
         }
-        __syncthreads();
+        __syncthreads();  // emitted: accumulators cleared, bitmap and wtot free
         if (threadIdx.x == 0) mark_done(dn, v);
-        if constexpr (BUILD) {  // cleared once every thread has emitted (the next row ORs into it)
-            if (threadIdx.x < NW) bits[threadIdx.x] = 0u;
-            __syncthreads();
-        }
         if (vn < 0) break;
-        j = jn;
+        j += G;
         v = vn;
         s0 = s0n;
         s1 = s1n;
@@ -2762,6 +2843,15 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
         lim = limn;
         lo = lon;
         nw = nwn;
+        vn = vnn;
+        if (vn >= 0) {
+            s0n = vbase[vn];
+            s1n = vbase[vn + 1];
+            o0n = vout[vn];
+            limn = vout[vn + 1];
+            lon = vlo[vn];
+            nwn = (vwid[vn] + 31) >> 5;
+        }
     }
 }
 
