@@ -255,6 +255,11 @@ int spgDestroy(spg_context* c) {
     if (c->pinned_small.p) (void)hipHostFree(c->pinned_small.p);
     for (auto& e : c->timer.ev)
         if (e) (void)hipEventDestroy(e);
+    for (int i = 0; i < 2; ++i) {
+        if (c->join_ev[i]) (void)hipEventDestroy(c->join_ev[i]);
+        if (c->aux[i]) (void)hipStreamDestroy(c->aux[i]);
+    }
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SPG_OK;

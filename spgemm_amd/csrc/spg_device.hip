@@ -3421,6 +3421,65 @@ static int launch_merge_class(spg_context* c, int b, const int* list, long long 
     return SPG_OK;
 }
 
+// The merge classes are independent (disjoint rows, disjoint output
+// ranges): the dense-tile class runs on the context's stream and the others
+// beside it on two more, so they fill the issue slots and the memory
+// pipeline its LDS-bound workgroups leave idle.  Forked from the context's
+// stream by an event and joined back before the phase ends (also inside a
+// graph capture).  false: everything stays on the context's stream.
+static bool aux_streams(spg_context* c) {
+    if (c->noalloc) return false;
+    if (!c->aux_tried) {
+        c->aux_tried = true;
+        bool ok = true;
+        for (int i = 0; i < 2 && ok; ++i)
+            ok = hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking) == hipSuccess;
+        ok = ok && hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) == hipSuccess;
+        for (int i = 0; i < 2 && ok; ++i)
+            ok = hipEventCreateWithFlags(&c->join_ev[i], hipEventDisableTiming) == hipSuccess;
+        if (!ok) (void)hipGetLastError();
+        c->aux_ok = ok;
+    }
+    return c->aux_ok;
+}
+
+static int fork_aux(spg_context* c) {
+    SPG_HIP(hipEventRecord(c->fork_ev, c->stream));
+    for (int i = 0; i < 2; ++i) SPG_HIP(hipStreamWaitEvent(c->aux[i], c->fork_ev, 0));
+    return SPG_OK;
+}
+
+static int join_aux(spg_context* c) {
+    for (int i = 0; i < 2; ++i) {
+        SPG_HIP(hipEventRecord(c->join_ev[i], c->aux[i]));
+        SPG_HIP(hipStreamWaitEvent(c->stream, c->join_ev[i], 0));
+    }
+    return SPG_OK;
+}
+
+// Launches the merge kernels of every class with rows, the dense-tile class
+// first (its workgroups take their LDS before the others fill the CUs).
+static int launch_merge_classes(spg_context* c, const int* vl, const long long* off,
+                                const MergeOut& o, bool build_all) {
+    const bool par = aux_streams(c);
+    if (par) SPG_TRY(fork_aux(c));
+    const hipStream_t main = c->stream;
+    int rc = SPG_OK;
+    const long long nt = off[kClassTile + 1] - off[kClassTile];
+    if (nt > 0) rc = launch_merge_class(c, kClassTile, vl + off[kClassTile], nt, o, build_all);
+    for (int b = kNumVClasses - 1; b >= 0 && rc == SPG_OK; --b) {
+        const long long cnt = off[b + 1] - off[b];
+        if (b == kClassTile || cnt <= 0) continue;
+        if (par) c->stream = (b == kClassTileM || b == kClassRankW) ? c->aux[0] : c->aux[1];
+        // without a symbolic pass every class builds its bitmaps; after one,
+        // the small tile rows still do (it kept none for them)
+        rc = launch_merge_class(c, b, vl + off[b], cnt, o, build_all || b == kClassTileS);
+        c->stream = main;
+    }
+    if (par) SPG_TRY(join_aux(c));
+    return rc;
+}
+
 int launch_merge(spg_context* c) {
     const long long V = c->V;
     SPG_TRY(fill<unsigned long long>(c, (unsigned long long*)c->counters.p, 0ull, 64));
@@ -3439,11 +3498,7 @@ int launch_merge(spg_context* c) {
     const MergeOut o{(const long long*)c->vout.p, (int*)c->ccol.p, (float*)c->cval.p, nullptr,
                      done_of(c)};
     tmark(c, SPG_PHASE_MERGE, 0);
-    for (int b = kNumVClasses - 1; b >= 0; --b) {
-        const long long cnt = off[b + 1] - off[b];
-        // small tile rows build their bitmap (the symbolic pass kept none)
-        if (cnt > 0) SPG_TRY(launch_merge_class(c, b, vl + off[b], cnt, o, b == kClassTileS));
-    }
+    SPG_TRY(launch_merge_classes(c, vl, off, o, false));
     tmark(c, SPG_PHASE_MERGE, 1);
     return SPG_OK;
 }
@@ -3839,10 +3894,7 @@ int launch_learned_merge(spg_context* c, spg_learned_stats* st) {
     const int* vl = (const int*)c->vlist.p;
     {
         const MergeOut o{pbase, (int*)c->pcol.p, (float*)c->pval.p, vcnt, Done{nullptr, nullptr}};
-        for (int b = kNumVClasses - 1; b >= 0; --b) {
-            const long long cnt = off[b + 1] - off[b];
-            if (cnt > 0) SPG_TRY(launch_merge_class(c, b, vl + off[b], cnt, o, true));
-        }
+        SPG_TRY(launch_merge_classes(c, vl, off, o, true));
     }
     // exact fallback for the rows that did not fit
     if (V > 0) {
@@ -3865,10 +3917,7 @@ int launch_learned_merge(spg_context* c, spg_learned_stats* st) {
         SPG_TRY(class_offsets(c, off));
         SPG_TRY(fill<unsigned long long>(c, ctr + kCtrRank, 0ull, 1));
         const MergeOut o{fbase, (int*)c->fcol.p, (float*)c->fval.p, vcnt, Done{nullptr, nullptr}};
-        for (int b = kNumVClasses - 1; b >= 0; --b) {
-            const long long cnt = off[b + 1] - off[b];
-            if (cnt > 0) SPG_TRY(launch_merge_class(c, b, vl + off[b], cnt, o, true));
-        }
+        SPG_TRY(launch_merge_classes(c, vl, off, o, true));
     }
     tmark(c, SPG_PHASE_MERGE, 1);
 

@@ -51,6 +51,12 @@ struct spg_context {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    // two more streams (default priority) on which merge classes run beside
+    // the dense-tile class, forked from and joined to `stream` by events;
+    // created on first use (never in the allocation-free mode)
+    hipStream_t aux[2] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[2] = {};
+    bool aux_tried = false, aux_ok = false;
     spg_config cfg{};
     std::string err;
 
