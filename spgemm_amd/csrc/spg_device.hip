@@ -568,10 +568,12 @@ __global__ void k_bsplit(const int* __restrict__ b_ptr, const int* __restrict__ 
     bsplit[q] = lo;
 }
 
-// Products of every tile of a long row: one workgroup per row; lane t of
-// every wave sums the tile-t pieces of its share of the row's B rows
-// (coalesced reads of the split points, four rows in flight), then the
-// waves' partial sums are added in LDS.
+// Products of every tile of a long row: one workgroup per row; the lanes of
+// a wave are (entry slot, tile) pairs, 64 / P entries side by side for a row
+// of at most P tiles (P a power of two), and sum the tile pieces of their
+// share of the row's B rows (coalesced reads of the split points, four
+// entries in flight per lane); the slots' and waves' partial sums are then
+// added by shuffles and in LDS.
 __global__ void k_tiled_key(int m, const int* __restrict__ vstart, int* __restrict__ key) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < m) key[i] = vstart[i + 1] - vstart[i] > 1 ? 0 : -1;
@@ -594,15 +596,21 @@ __global__ __launch_bounds__(kBlock) void k_row_tile_bound(const int* __restrict
         const int vs = vstart[i];
         const int Ti = vstart[i + 1] - vs;  // the row's tiles (T + 1: split-point stride)
         const int e0 = rptr[i], e1 = rptr[i + 1];
-        for (int t0 = 0; t0 < Ti; t0 += 64) {
-            const int t = t0 + lane;
+        // a wave covers G = 64 / P entries at a time, P = the tiles rounded
+        // up to a power of two (at most 64): lane = (entry slot, tile)
+        const int lp = Ti >= 64 ? 6 : 32 - __clz(Ti - 1);
+        const int P = 1 << lp, G = 64 >> lp;
+        const int g = lane >> lp, tl = lane & (P - 1);
+        const int stp = NWV * G;
+        for (int t0 = 0; t0 < Ti; t0 += P) {
+            const int t = t0 + tl;
             long long s = 0;
             if (t < Ti) {
-                int e = e0 + wv;
-                for (; e + 3 * NWV < e1; e += 4 * NWV) {
+                int e = e0 + wv * G + g;
+                for (; e + 3 * stp < e1; e += 4 * stp) {
                     int kk[4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) kk[u] = rk[e + u * NWV];
+                    for (int u = 0; u < 4; ++u) kk[u] = rk[e + u * stp];
                     int d[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
@@ -612,14 +620,15 @@ __global__ __launch_bounds__(kBlock) void k_row_tile_bound(const int* __restrict
 #pragma unroll
                     for (int u = 0; u < 4; ++u) s += d[u];
                 }
-                for (; e < e1; e += NWV) {
+                for (; e < e1; e += stp) {
                     const int* bs = bsplit + (long long)rk[e] * (T + 1);
                     s += bs[t + 1] - bs[t];
                 }
             }
+            for (int o = P; o < 64; o <<= 1) s += __shfl_xor(s, o, 64);  // over the entry slots
             part[wv][lane] = s;
             __syncthreads();
-            if (wv == 0 && t < Ti) {
+            if (wv == 0 && g == 0 && t < Ti) {
                 long long tot = 0;
 #pragma unroll
                 for (int u = 0; u < NWV; ++u) tot += part[u][lane];
