@@ -2388,7 +2388,7 @@ __global__ __launch_bounds__(64 * WPB) void k_symbolic_hash_wave(
 // slots compacted, and each key's output position found by counting the
 // smaller keys (at most 256 x 256 comparisons of broadcast LDS reads).
 // cnt_out as in k_merge_hash (-1: the predicted table size was too small).
-template <int LOG2H, int WPB>
+template <int LOG2H, int WPB, bool STAMP>
 __global__ __launch_bounds__(64 * WPB) void k_merge_hash_wave(
     const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
     const int* __restrict__ icol, const float* __restrict__ ival,
@@ -2470,7 +2470,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_hash_wave(
             }
         }
         if (lane == 0 && cnt_out) cnt_out[v] = total;
-        if (lane == 0) mark_done(dn, v);
+        if (STAMP && lane == 0) mark_done(dn, v);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
     }
@@ -2595,7 +2595,7 @@ __global__ void k_vclass_merge(long long V, const long long* __restrict__ vcnt,
 // cnt_out (may be null) receives the row's distinct-column count, or -1
 // when the table (sized from a predicted count) could not hold the row; the
 // row's output is then incomplete and the caller merges it again.
-template <int LOG2H>
+template <int LOG2H, bool STAMP>
 __global__ __launch_bounds__(kBlock) void k_merge_hash(
     const int* __restrict__ vlist, const long long* __restrict__ vbase,
     const int* __restrict__ icol, const float* __restrict__ ival,
@@ -2676,7 +2676,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_hash(
         ccol[o + q] = skey[q];
         cval[o + q] = (float)sval[q];
     }
-    if (threadIdx.x == 0) mark_done(dn, v);
+    if (STAMP && threadIdx.x == 0) mark_done(dn, v);
 }
 
 // Tile-width virtual rows: dense FP64 accumulator over the tile's columns
@@ -2691,7 +2691,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_hash(
 // rank is the wave's base plus the mask bits below its lane, and the set
 // columns are written to consecutive output positions.  Only touched
 // accumulators are cleared (while being emitted).
-template <bool BUILD>
+template <bool BUILD, bool STAMP>
 __global__ __launch_bounds__(kPT) void k_merge_tile(
     const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
     const int* __restrict__ vlo, const int* __restrict__ vwid,
@@ -2842,7 +2842,7 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
             ob += __popcll(m);
         }
         __syncthreads();  // emitted: accumulators cleared, bitmap and wtot free
-        if (threadIdx.x == 0) mark_done(dn, v);
+        if (STAMP && threadIdx.x == 0) mark_done(dn, v);
         if (vn < 0) break;
         j += G;
         v = vn;
@@ -2872,7 +2872,7 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
 // col[rank], and the row is written out with fully coalesced stores.  With
 // BUILD, cnt_out receives the row's distinct-column count (which may exceed
 // its output capacity; only the capacity is written).
-template <int CAP, int WPB, bool BUILD>
+template <int CAP, int WPB, bool BUILD, bool STAMP>
 __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
     const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
     const int* __restrict__ vlo, const unsigned* __restrict__ tilebits,
@@ -2986,7 +2986,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
             ccol[o0 + q] = colr[q];
             cval[o0 + q] = (float)acc[q];
         }
-        if (lane == 0) mark_done(dn, v);
+        if (STAMP && lane == 0) mark_done(dn, v);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
     }
@@ -2998,6 +2998,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
 // columns are emitted from the bitmap in ascending order (no sort).
 constexpr int kRankChunk = 8;  // rows taken per counter increment
 
+template <bool STAMP>
 __global__ __launch_bounds__(kPT) void k_merge_rank(
     const int* __restrict__ vlist, long long nlist, unsigned long long* __restrict__ ctr,
     const long long* __restrict__ vbase, const int* __restrict__ vlo,
@@ -3122,7 +3123,7 @@ __global__ __launch_bounds__(kPT) void k_merge_rank(
                 }
             }
             __syncthreads();
-            if (threadIdx.x == 0) mark_done(dn, v);
+            if (STAMP && threadIdx.x == 0) mark_done(dn, v);
         }
     }
 }
@@ -3176,7 +3177,7 @@ __device__ __forceinline__ int rank_of(const unsigned* bits, const unsigned shor
     return (int)pre[w >> 1] + ((w & 1) ? __popc(bb.x) + __popc(bb.y & low) : __popc(bb.x & low));
 }
 
-template <int NW, int CAP>
+template <int NW, int CAP, bool STAMP>
 __global__ __launch_bounds__(64) void k_merge_rank_wave(
     const int* __restrict__ vlist, long long nlist, int G, const long long* __restrict__ vbase,
     const int* __restrict__ vlo, const int* __restrict__ vwid, const int* __restrict__ icol,
@@ -3309,7 +3310,7 @@ __global__ __launch_bounds__(64) void k_merge_rank_wave(
                 ccol[o0 + q] = colr[q];
                 cval[o0 + q] = (float)acc[q];
             }
-            if (lane == 0) mark_done(dn, v);
+            if (STAMP && lane == 0) mark_done(dn, v);
             for (int w = w0; w < w1; ++w) bits[w] = 0u;  // a lane clears the words it counted
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -3338,15 +3339,16 @@ static Done done_of(const spg_context* c) {
     return d;
 }
 
-template <int LOG2H>
+template <int LOG2H, bool STAMP>
 static void launch_merge_hash(spg_context* c, const int* list, long long cnt, const MergeOut& o) {
-    hipLaunchKernelGGL(k_merge_hash<LOG2H>, dim3((unsigned)cnt), dim3(kBlock), 0, c->stream,
+    hipLaunchKernelGGL((k_merge_hash<LOG2H, STAMP>), dim3((unsigned)cnt), dim3(kBlock), 0, c->stream,
                        list, (const long long*)c->vbase.p, (const int*)inter_cols(c),
                        (const float*)inter_vals(c), o.obase, o.ocol, o.oval, o.cnt, o.dn);
 }
 
 // One merge kernel over the virtual rows list[0..cnt) of class b.  build:
 // tile rows build their column bitmap themselves (no symbolic pass ran).
+template <bool STAMP>
 static int launch_merge_class(spg_context* c, int b, const int* list, long long cnt,
                               const MergeOut& o, bool build) {
     const long long* vbase = (const long long*)c->vbase.p;
@@ -3357,42 +3359,42 @@ static int launch_merge_class(spg_context* c, int b, const int* list, long long 
     if (b == kClassTileS) {
         const dim3 g(persistent_grid(c, (cnt + 7) / 8, 4));
         if (build)
-            hipLaunchKernelGGL((k_merge_tile_wave<256, 8, true>), g, dim3(512), 0, c->stream, list,
+            hipLaunchKernelGGL((k_merge_tile_wave<256, 8, true, STAMP>), g, dim3(512), 0, c->stream, list,
                                cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
         else
-            hipLaunchKernelGGL((k_merge_tile_wave<256, 8, false>), g, dim3(512), 0, c->stream, list,
+            hipLaunchKernelGGL((k_merge_tile_wave<256, 8, false, STAMP>), g, dim3(512), 0, c->stream, list,
                                cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
     } else if (b == kClassTileM) {
         // one wave per workgroup: 13.8 KB of LDS each, so a CU holds 11 of
         // them (four-wave workgroups fit only twice: 8 waves)
         const dim3 g(persistent_grid(c, cnt, 11));
         if (build)
-            hipLaunchKernelGGL((k_merge_tile_wave<1024, 1, true>), g, dim3(64), 0, c->stream, list,
+            hipLaunchKernelGGL((k_merge_tile_wave<1024, 1, true, STAMP>), g, dim3(64), 0, c->stream, list,
                                cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
         else
-            hipLaunchKernelGGL((k_merge_tile_wave<1024, 1, false>), g, dim3(64), 0, c->stream,
+            hipLaunchKernelGGL((k_merge_tile_wave<1024, 1, false, STAMP>), g, dim3(64), 0, c->stream,
                                list, cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval,
                                o.cnt, o.dn);
     } else if (b == kClassTile) {
         const dim3 g(persistent_grid(c, cnt, 2));
         if (build)
-            hipLaunchKernelGGL(k_merge_tile<true>, g, dim3(kPT), 0, c->stream, list, cnt, vbase,
+            hipLaunchKernelGGL((k_merge_tile<true, STAMP>), g, dim3(kPT), 0, c->stream, list, cnt, vbase,
                                vlo, (const int*)c->vwid.p, tb, icol, ival, o.obase, o.ocol,
                                o.oval, o.cnt, o.dn);
         else
-            hipLaunchKernelGGL(k_merge_tile<false>, g, dim3(kPT), 0, c->stream, list, cnt, vbase,
+            hipLaunchKernelGGL((k_merge_tile<false, STAMP>), g, dim3(kPT), 0, c->stream, list, cnt, vbase,
                                vlo, (const int*)c->vwid.p, tb, icol, ival, o.obase, o.ocol,
                                o.oval, o.cnt, o.dn);
     } else if (b == kClassRankW) {
         // rows per wave: 64 when the list fills 7 waves per CU that many times
         const long long waves = (long long)c->num_cu * 7;
         const int G = (int)std::min<long long>(64, std::max<long long>(1, cnt / waves));
-        hipLaunchKernelGGL((k_merge_rank_wave<kRankWaveWords, kRankWaveCap>),
+        hipLaunchKernelGGL((k_merge_rank_wave<kRankWaveWords, kRankWaveCap, STAMP>),
                            dim3(persistent_grid(c, (cnt + G - 1) / G, 7)), dim3(64), 0,
                            c->stream, list, cnt, G, vbase, vlo, (const int*)c->vwid.p, icol, ival,
                            o.obase, o.ocol, o.oval, o.cnt, o.dn);
     } else if (b == kClassRank) {
-        hipLaunchKernelGGL(k_merge_rank,
+        hipLaunchKernelGGL(k_merge_rank<STAMP>,
                            dim3(persistent_grid(c, (cnt + kRankChunk - 1) / kRankChunk, 2)),
                            dim3(kPT), 0, c->stream, list, cnt,
                            (unsigned long long*)c->counters.p + 2, vbase, vlo, icol, ival,
@@ -3402,32 +3404,41 @@ static int launch_merge_class(spg_context* c, int b, const int* list, long long 
         const dim3 bw(64 * kHashWaveWPB);
         switch (b + kHashMinLog2) {
             case 6:
-                hipLaunchKernelGGL((k_merge_hash_wave<6, kHashWaveWPB>), gw, bw, 0, c->stream,
+                hipLaunchKernelGGL((k_merge_hash_wave<6, kHashWaveWPB, STAMP>), gw, bw, 0, c->stream,
                                    list, cnt, vbase, icol, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
                 break;
             case 7:
-                hipLaunchKernelGGL((k_merge_hash_wave<7, kHashWaveWPB>), gw, bw, 0, c->stream,
+                hipLaunchKernelGGL((k_merge_hash_wave<7, kHashWaveWPB, STAMP>), gw, bw, 0, c->stream,
                                    list, cnt, vbase, icol, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
                 break;
             case 8:
-                hipLaunchKernelGGL((k_merge_hash_wave<8, kHashWaveWPB>), gw, bw, 0, c->stream,
+                hipLaunchKernelGGL((k_merge_hash_wave<8, kHashWaveWPB, STAMP>), gw, bw, 0, c->stream,
                                    list, cnt, vbase, icol, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
                 break;
             case 9:
-                hipLaunchKernelGGL((k_merge_hash_wave<9, 4>),
+                hipLaunchKernelGGL((k_merge_hash_wave<9, 4, STAMP>),
                                    dim3(persistent_grid(c, (cnt + 3) / 4, 4)), dim3(256), 0,
                                    c->stream, list, cnt, vbase, icol, ival, o.obase, o.ocol,
                                    o.oval, o.cnt, o.dn);
                 break;
-            case 10: launch_merge_hash<10>(c, list, cnt, o); break;
-            case 11: launch_merge_hash<11>(c, list, cnt, o); break;
-            case 12: launch_merge_hash<12>(c, list, cnt, o); break;
-            case 13: launch_merge_hash<13>(c, list, cnt, o); break;
+            case 10: launch_merge_hash<10, STAMP>(c, list, cnt, o); break;
+            case 11: launch_merge_hash<11, STAMP>(c, list, cnt, o); break;
+            case 12: launch_merge_hash<12, STAMP>(c, list, cnt, o); break;
+            case 13: launch_merge_hash<13, STAMP>(c, list, cnt, o); break;
             default: return fail(c, SPG_EINVAL, "bad merge class");
         }
     }
     SPG_HIP(hipGetLastError());
     return SPG_OK;
+}
+
+// The kernels that stamp completion times are separate instances: a stamp
+// that is compiled in but never taken still slowed the wave hash merge by
+// 70 % (1.0 -> 1.75 ms on the 64^3 stencil).
+static int merge_class(spg_context* c, int b, const int* list, long long cnt, const MergeOut& o,
+                       bool build) {
+    return o.dn.t ? launch_merge_class<true>(c, b, list, cnt, o, build)
+                  : launch_merge_class<false>(c, b, list, cnt, o, build);
 }
 
 // The merge classes are independent (disjoint rows, disjoint output
@@ -3468,21 +3479,25 @@ static int join_aux(spg_context* c) {
 
 // Launches the merge kernels of every class with rows, the dense-tile class
 // first (its workgroups take their LDS before the others fill the CUs).
+// Products below this many intermediate products merge on one stream: the
+// fork and join cost tens of microseconds, more than the overlap saves.
+constexpr long long kForkMinProducts = 1LL << 26;
+
 static int launch_merge_classes(spg_context* c, const int* vl, const long long* off,
                                 const MergeOut& o, bool build_all) {
-    const bool par = aux_streams(c);
+    const bool par = c->nprod >= kForkMinProducts && aux_streams(c);
     if (par) SPG_TRY(fork_aux(c));
     const hipStream_t main = c->stream;
     int rc = SPG_OK;
     const long long nt = off[kClassTile + 1] - off[kClassTile];
-    if (nt > 0) rc = launch_merge_class(c, kClassTile, vl + off[kClassTile], nt, o, build_all);
+    if (nt > 0) rc = merge_class(c, kClassTile, vl + off[kClassTile], nt, o, build_all);
     for (int b = kNumVClasses - 1; b >= 0 && rc == SPG_OK; --b) {
         const long long cnt = off[b + 1] - off[b];
         if (b == kClassTile || cnt <= 0) continue;
         if (par) c->stream = (b == kClassTileM || b == kClassRankW) ? c->aux[0] : c->aux[1];
         // without a symbolic pass every class builds its bitmaps; after one,
         // the small tile rows still do (it kept none for them)
-        rc = launch_merge_class(c, b, vl + off[b], cnt, o, build_all || b == kClassTileS);
+        rc = merge_class(c, b, vl + off[b], cnt, o, build_all || b == kClassTileS);
         c->stream = main;
     }
     if (par) SPG_TRY(join_aux(c));
@@ -3672,6 +3687,7 @@ __global__ void k_learned_check(long long V, const long long* __restrict__ vcnt,
 // one row's descriptor, and every output position of the range finds its row
 // by a binary search over the 64 row offsets (shuffles), so stores are
 // coalesced and no per-row latency chain is serialized.
+template <bool STAMP>
 __global__ __launch_bounds__(kPT) void k_learned_compact(
     long long V, const long long* __restrict__ vcnt, const long long* __restrict__ vout,
     const long long* __restrict__ pbase, const long long* __restrict__ fbase,
@@ -3763,7 +3779,7 @@ __global__ __launch_bounds__(kPT) void k_learned_compact(
                     }
                 }
             }
-        if (dn.t) {  // uniform: this group's rows are written; one atomic per product
+        if (STAMP && dn.t) {  // uniform: this group's rows are written; one atomic per product
             const int pv = v < V && cnt > 0 ? dn.vprod[v] : -1;
             const int pp = __shfl_up(pv, 1, 64);
             if (pv >= 0 && (lane == 0 || pp != pv))
@@ -3833,7 +3849,7 @@ __global__ __launch_bounds__(kPT) void k_learned_compact(
                 }
             }
         }
-        if (dn.t) {  // uniform: this group's rows are written; one atomic per product
+        if (STAMP && dn.t) {  // uniform: this group's rows are written; one atomic per product
             const int pv = v < V && cnt > 0 ? dn.vprod[v] : -1;
             const int pp = __shfl_up(pv, 1, 64);
             if (pv >= 0 && (lane == 0 || pp != pv))
@@ -3941,12 +3957,14 @@ int launch_learned_merge(spg_context* c, spg_learned_stats* st) {
     if (V > 0) {
         const int* fcol = nover > 0 ? (const int*)c->fcol.p : nullptr;
         const float* fval = nover > 0 ? (const float*)c->fval.p : nullptr;
-        hipLaunchKernelGGL(k_learned_compact,
+        const Done dn = done_of(c);
+        auto compact = dn.t ? k_learned_compact<true> : k_learned_compact<false>;
+        hipLaunchKernelGGL(compact,
                            dim3(persistent_grid(c, (V + 64 * 8 - 1) / (64 * 8), 4)), dim3(kPT),
                            0, c->stream, V, (const long long*)vcnt, (const long long*)c->vout.p,
                            (const long long*)pbase, (const long long*)fbase,
                            (const int*)c->pcol.p, (const float*)c->pval.p, fcol, fval,
-                           (int*)c->ccol.p, (float*)c->cval.p, ctr + kCtrUnrec, done_of(c));
+                           (int*)c->ccol.p, (float*)c->cval.p, ctr + kCtrUnrec, dn);
         SPG_HIP(hipGetLastError());
     }
     tmark(c, SPG_PHASE_COMPACT, 1);
