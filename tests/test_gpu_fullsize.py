@@ -1,6 +1,9 @@
-"""The benchmark workloads at their full sizes, checked through properties
-that do not need the oracle to run the whole product (it would take minutes
-to hours):
+"""The benchmark workloads at their full sizes.  Sampled rows (the hub rows
+with the largest bounds, rows whose tiles straddle tile boundaries, seeded
+random rows) are compared with the oracle's row-wise product
+(oracle.gustavson(rows=...)): structure bit-exact, values within VAL_RTOL.
+The whole products are checked through properties that do not need the
+oracle to run all of them (it would take minutes to hours):
 
 * structure invariants: row pointers non-decreasing, columns in range and
   strictly increasing inside every row, nnz(C_i) <= min(ub_i, n), and
@@ -127,16 +130,18 @@ BENCH_CFG = dict(tile_log2=13, item_products=8192, long_row_ub=512, xcd_swizzle=
                  heavy_factor=50, rank_min_count=1024)
 
 
-def test_rmat_s18_bench_config_vs_oracle(spg, orc, gpu_ctx):
+@pytest.mark.parametrize("rank_min", [1024, 512, 128])
+def test_rmat_s18_bench_config_vs_oracle(spg, orc, gpu_ctx, rank_min):
     """R-MAT scale 18, edge factor 16 with the settings the benchmark times
-    (row-chunk expansion at 32 tiles per long row, long_row_ub 512,
-    rank_min_count 1024): about 2k rows of the plain and of the learned C
-    against the oracle, the hub rows (up to 2.5M products, 32 tiles) among
-    them."""
-    print("case full R-MAT s18 ef16, benchmark settings, rows vs oracle", flush=True)
+    (row-chunk expansion at 32 tiles per long row, long_row_ub 512, and
+    each rank_min_count the tuner has chosen for the plain or the learned
+    path): about 2k rows of the plain and of the learned C against the
+    oracle, the hub rows (up to 2.5M products, 32 tiles) among them."""
+    print(f"case full R-MAT s18 ef16, benchmark settings, rank_min_count {rank_min}, "
+          "rows vs oracle", flush=True)
     ctx = gpu_ctx
     A = spg.gen_rmat(18, 18, 16 << 18, seed=0)
-    ctx.set_config(expand_mode=spg.EXPAND_ROWS, **BENCH_CFG)
+    ctx.set_config(expand_mode=spg.EXPAND_ROWS, **dict(BENCH_CFG, rank_min_count=rank_min))
     ctx.set_operands_from_coo(A, A)
     ctx.multiply()
     ub = ctx.upper_bound(A.num_rows)
