@@ -515,7 +515,7 @@ int spgInitGEMM(spg_context* c, cm* C, int64_t* nprod) {
                         "split the product");
     const int64_t np = lim[3];
     c->nprod = np;
-    SPG_TRY(ensure(c, c->inter, sizeof(int32_t) * inter_words(np)));
+    SPG_TRY(ensure(c, c->inter, inter_bytes(np)));
     tmark(c, SPG_PHASE_INIT, 1);
     if (C) {
         cmInit(C);
@@ -749,6 +749,23 @@ int spgGetIntermediateByRow(spg_context* c, int64_t* row_ptr, int32_t* cols, flo
             const long long q = islot(s);
             if (cols) cols[s] = w[(size_t)q];
             if (vals) memcpy(&vals[s], &w[(size_t)(q + kIGroup)], sizeof(float));
+        }
+        if (cols && np > 0 && c->V > 0) {  // narrow rows: 16-bit columns, lo + (c16 - lo) mod 2^16
+            const size_t V = (size_t)c->V;
+            std::vector<uint16_t> c16((size_t)np);
+            std::vector<int64_t> vb(V), vu(V);
+            std::vector<int32_t> vl(V), vw(V);
+            SPG_TRY(copy_d2h(c, c16.data(), inter_cols16(c), sizeof(uint16_t) * c16.size()));
+            SPG_TRY(copy_d2h(c, vb.data(), c->vbase.p, sizeof(int64_t) * V));
+            SPG_TRY(copy_d2h(c, vu.data(), c->vub.p, sizeof(int64_t) * V));
+            SPG_TRY(copy_d2h(c, vl.data(), c->vlo.p, sizeof(int32_t) * V));
+            SPG_TRY(copy_d2h(c, vw.data(), c->vwid.p, sizeof(int32_t) * V));
+            for (size_t v = 0; v < V; ++v) {
+                if (vw[v] > c->W) continue;
+                const uint32_t lo = (uint32_t)vl[v];
+                for (int64_t s = vb[v]; s < vb[v] + vu[v]; ++s)
+                    cols[s] = (int32_t)(lo + ((c16[(size_t)s] - lo) & 0xFFFFu));
+            }
         }
     }
     return SPG_OK;

@@ -45,6 +45,16 @@ __device__ __forceinline__ unsigned long long lanemask_lt() {
     return (1ull << lane_id()) - 1ull;
 }
 
+// Four columns of a narrow row, relative to its origin lo (0 .. width-1),
+// from the 16-bit column array (q: products b..b+3, one 8-byte load).
+__device__ __forceinline__ void rel4(uint2 q, int lo, int (&c)[4]) {
+    const unsigned l = (unsigned)lo;
+    c[0] = (int)((q.x - l) & 0xFFFFu);
+    c[1] = (int)(((q.x >> 16) - l) & 0xFFFFu);
+    c[2] = (int)((q.y - l) & 0xFFFFu);
+    c[3] = (int)(((q.y >> 16) - l) & 0xFFFFu);
+}
+
 // Exclusive scan over a 256-thread workgroup.  sh needs kBlock/64 slots.
 template <typename T>
 __device__ __forceinline__ T block_exscan(T x, T* total, T* sh) {
@@ -1090,12 +1100,14 @@ struct ExpandArgs {
     int P;                      // products per work item
     const int* vstart;
     const long long* vbase;
+    const int* vwid;
     int* cursor;
     const int* bsplit;
     const int2* kwin;  // column window of every k (batched products), or null
     int T, tile_log2;
     int swizzle;
     int* icol;
+    uint16_t* icol16;  // columns of narrow rows (width <= the tile width)
     float* ival;
     const int* rcskip;  // rows expanded in chunks (key >= 0) are skipped, or null
 };
@@ -1167,7 +1179,7 @@ __device__ void expand_work(const ExpandArgs& g, ExpandLds& L, int k, long long 
             if (g.vstart[i + 1] - vs == 1) {
                 const int off = atomicAdd(&g.cursor[vs], nj);
                 L.sbase[q] = g.vbase[vs] + off;
-                L.sflag[q] = 0;
+                L.sflag[q] = g.vwid[vs] <= (1 << g.tile_log2) ? 2 : 0;  // 2: narrow
             } else {
                 L.sbase[q] = vs;
                 L.sflag[q] = 1;
@@ -1177,7 +1189,7 @@ __device__ void expand_work(const ExpandArgs& g, ExpandLds& L, int k, long long 
         if (T > 1) {
             for (int pr = threadIdx.x; pr < cnt * T; pr += kBlock) {
                 const int q = pr / T, t = pr - q * T;
-                if (!L.sflag[q]) continue;
+                if (L.sflag[q] != 1) continue;
                 const int s = max(L.sbs[t], jlo), e2 = min(L.sbs[t + 1], jhi);
                 if (e2 > s) {
                     const int v = (int)L.sbase[q] + t;
@@ -1201,17 +1213,21 @@ __device__ void expand_work(const ExpandArgs& g, ExpandLds& L, int k, long long 
             const int* __restrict__ bidx = g.b_idx;
             const float* __restrict__ bval = g.b_val;
             int* __restrict__ icol = g.icol;
+            uint16_t* __restrict__ icol16 = g.icol16;
             float* __restrict__ ival = g.ival;
+            // fl: 0 an untiled row, 1 a tile of a tiled row, 2 an untiled
+            // narrow row; tiles and narrow rows take 16-bit columns
             if (breg) {
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int jj = lj + u * G;
                     if (jj < nj) {
                         long long slot;
-                        if (!fl) slot = base + jj;
+                        if (fl != 1) slot = base + jj;
                         else slot = L.tbase[q * T + (bc[u] >> g.tile_log2) - tfirst] + jlo + jj;
                         const long long w = islot(slot);
-                        icol[w] = bc[u];
+                        if (fl) icol16[slot] = (uint16_t)bc[u];
+                        else icol[w] = bc[u];
                         ival[w] = __fmul_rn(a, bx[u]);
                     }
                 }
@@ -1231,10 +1247,11 @@ __device__ void expand_work(const ExpandArgs& g, ExpandLds& L, int k, long long 
                     const int jj = jj0 + u * G;
                     if (jj < nj) {
                         long long slot;
-                        if (!fl) slot = base + jj;
+                        if (fl != 1) slot = base + jj;
                         else slot = L.tbase[q * T + (col[u] >> g.tile_log2) - tfirst] + jlo + jj;
                         const long long w = islot(slot);
-                        icol[w] = col[u];
+                        if (fl) icol16[slot] = (uint16_t)col[u];
+                        else icol[w] = col[u];
                         ival[w] = __fmul_rn(a, bv[u]);
                     }
                 }
@@ -1335,6 +1352,7 @@ static ExpandArgs expand_args(spg_context* c) {
     g.P = c->cfg.item_products;
     g.vstart = (const int*)c->vstart.p;
     g.vbase = (const long long*)c->vbase.p;
+    g.vwid = (const int*)c->vwid.p;
     g.cursor = (int*)c->cursor.p;
     g.bsplit = (const int*)c->bsplit.p;
     g.kwin = (const int2*)c->kwin;
@@ -1342,6 +1360,7 @@ static ExpandArgs expand_args(spg_context* c) {
     g.tile_log2 = c->cfg.tile_log2;
     g.swizzle = c->cfg.xcd_swizzle;
     g.icol = inter_cols(c);
+    g.icol16 = inter_cols16(c);
     g.ival = inter_vals(c);
     return g;
 }
@@ -1396,8 +1415,10 @@ struct RowsArgs {
     const int* vrow_row;
     const int* vstart;
     const long long* vbase;
-    int T;
+    const int* vwid;
+    int T, W;
     int* icol;
+    uint16_t* icol16;  // columns of narrow rows (width <= W)
     float* ival;
 };
 
@@ -1421,12 +1442,14 @@ __global__ __launch_bounds__(kPT) void k_expand_rows_wave(RowsArgs g, const int*
     const long long stride = (long long)gridDim.x * NWV;
     const int2* __restrict__ bpair = g.bpair;
     int* __restrict__ icol = g.icol;
+    uint16_t* __restrict__ icol16 = g.icol16;
     float* __restrict__ ival = g.ival;
     for (long long j = (long long)blockIdx.x * NWV + wv; j < nlist; j += stride) {
         const int v = vlist[j];
         const int i = g.vrow_row[v];
         const int vs = g.vstart[i];
         const bool tiled = g.vstart[i + 1] - vs > 1;
+        const bool nar = g.vwid[v] <= g.W;
         const int t = v - vs;
         const int e1 = g.rptr[i + 1];
         long long dst = g.vbase[v];
@@ -1483,7 +1506,8 @@ __global__ __launch_bounds__(kPT) void k_expand_rows_wave(RowsArgs g, const int*
                 for (int u = 0; u < 4; ++u) {
                     if (p[u] < total) {
                         const long long w = islot(dst + p[u]);
-                        icol[w] = col[u];
+                        if (nar) icol16[dst + p[u]] = (uint16_t)col[u];
+                        else icol[w] = col[u];
                         ival[w] = __fmul_rn(sa[u], bv[u]);
                     }
                 }
@@ -1507,6 +1531,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_rows_wg(RowsArgs g, const int
         const int i = g.vrow_row[v];
         const int vs = g.vstart[i];
         const bool tiled = g.vstart[i + 1] - vs > 1;
+        const bool nar = g.vwid[v] <= g.W;
         const int t = v - vs;
         const int e1 = g.rptr[i + 1];
         long long dst = g.vbase[v];
@@ -1551,7 +1576,8 @@ __global__ __launch_bounds__(kBlock) void k_expand_rows_wg(RowsArgs g, const int
                 for (int u = 0; u < 4; ++u) {
                     if (p[u] < total) {
                         const long long w = islot(dst + p[u]);
-                        g.icol[w] = col[u];
+                        if (nar) g.icol16[dst + p[u]] = (uint16_t)col[u];
+                        else g.icol[w] = col[u];
                         g.ival[w] = __fmul_rn(sa[lo[u]], bv[u]);
                     }
                 }
@@ -1602,7 +1628,7 @@ struct RCArgs {
     const long long* vbase;
     int* cursor;
     int T;
-    int* icol;
+    uint16_t* icol16;  // tiles are narrow rows: 16-bit columns
     float* ival;
 };
 
@@ -1771,11 +1797,13 @@ __global__ __launch_bounds__(kBlock) void k_expand_rowchunks(RCArgs g, const int
                 }
 #pragma unroll
                 for (int v = 0; v < kRCU; ++v) {
-                    // slot s0 + d sits at word (s0 >> 6) * 128 + o + (o & 64), o = (s0 & 63) + d
+                    // slot s0 + d: its value at word (s0 >> 6) * 128 + o + (o & 64) of
+                    // the value array, o = (s0 & 63) + d; its 16-bit column at s0 + d
                     const long long w0 = (s0[v] >> kIGroupLog2) << (kIGroupLog2 + 1);
-                    const int o = (int)(s0[v] & (kIGroup - 1)) + (pos[v] - boff[v]);
+                    const int d = pos[v] - boff[v];
+                    const int o = (int)(s0[v] & (kIGroup - 1)) + d;
                     const unsigned off = (unsigned)(o + (o & kIGroup)) << 2;
-                    *(int*)((char*)(g.icol + w0) + off) = col[v];
+                    *(uint16_t*)((char*)(g.icol16 + s0[v]) + ((unsigned)d << 1)) = (uint16_t)col[v];
                     *(float*)((char*)(g.ival + w0) + off) = val[v];
                 }
             }
@@ -1800,15 +1828,21 @@ __global__ void k_interleave(const int* __restrict__ idx, const float* __restric
 // binary search over the chunk's product offsets, as the row-chunk kernel
 // does per tile.  One workgroup per chunk amortizes the per-row chain of
 // dependent loads that a wave per row pays.
+// An entry's base carries kNarrowBit when its row is narrow (width <= W:
+// 16-bit columns).
+constexpr long long kNarrowBit = 1LL << 62;
+
 __global__ void k_expand_entry_base(int m, const int* __restrict__ rptr, const int* __restrict__ rk,
                                     const int* __restrict__ b_ptr, const int* __restrict__ vstart,
                                     const long long* __restrict__ vbase,
+                                    const int* __restrict__ vwid, int W,
                                     long long* __restrict__ ebase) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const int vs = vstart[i];
     if (vstart[i + 1] - vs != 1) return;  // a tiled row: its entries keep -1
-    long long run = vbase[vs];
+    // slots stay below 2^62, so the flag does not touch them
+    long long run = vbase[vs] | (vwid[vs] <= W ? kNarrowBit : 0LL);
     const int e1 = rptr[i + 1];
     int e = rptr[i];
     for (; e + 4 <= e1; e += 4) {  // four entries' loads in flight
@@ -1837,6 +1871,7 @@ struct EntryArgs {
     const int2* bpair;
     const long long* ebase;
     int* icol;
+    uint16_t* icol16;
     float* ival;
 };
 
@@ -1845,23 +1880,27 @@ __global__ __launch_bounds__(kBlock) void k_expand_entrychunks(EntryArgs g, long
     __shared__ int pref[kBlock];    // each entry's offset in the chunk's products
     __shared__ long long sbo[kBlock];  // slot of product offset 0 for the entry
     __shared__ float sa[kBlock];
+    __shared__ int snar[kBlock];       // the entry's row is narrow (16-bit columns)
     __shared__ int sh[kBlock / 64];
     constexpr int NW = kBlock / 64;
     const int wv = threadIdx.x >> 6, lane = lane_id();
     const int2* __restrict__ bpair = g.bpair;
     int* __restrict__ icol = g.icol;
+    uint16_t* __restrict__ icol16 = g.icol16;
     float* __restrict__ ival = g.ival;
     const long long nunits = (nent + kBlock - 1) / kBlock;
     for (long long u = blockIdx.x; u < nunits; u += gridDim.x) {
         const long long c0 = u * kBlock;
         const int ne = (int)min((long long)kBlock, nent - c0);
-        int len = 0, x = 0;
+        int len = 0, x = 0, nar = 0;
         long long base = -1;
         float a = 0.f;
         if (threadIdx.x < ne) {
             const long long e = c0 + threadIdx.x;
             base = g.ebase[e];
             if (base >= 0) {  // an entry of an untiled row
+                nar = (base & kNarrowBit) != 0;
+                base &= kNarrowBit - 1;
                 const int k = g.rk[e];
                 x = g.b_ptr[k];
                 len = g.b_ptr[k + 1] - x;
@@ -1875,6 +1914,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_entrychunks(EntryArgs g, long
             pref[threadIdx.x] = off;
             sbo[threadIdx.x] = base - off;
             sa[threadIdx.x] = a;
+            snar[threadIdx.x] = nar;
         }
         __syncthreads();
         const int nbk = (sum + 63) >> 6;
@@ -1905,8 +1945,10 @@ __global__ __launch_bounds__(kBlock) void k_expand_entrychunks(EntryArgs g, long
             }
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const long long w = islot(sbo[ev[v]] + pos[v]);
-                icol[w] = col[v];
+                const long long sl = sbo[ev[v]] + pos[v];
+                const long long w = islot(sl);
+                if (snar[ev[v]]) icol16[sl] = (uint16_t)col[v];
+                else icol[w] = col[v];
                 ival[w] = __fmul_rn(sa[ev[v]], bv[v]);
             }
         }
@@ -1998,7 +2040,7 @@ static int launch_rowchunks(spg_context* c, const RCPlan& pl) {
     h.vbase = (const long long*)c->vbase.p;
     h.cursor = (int*)c->cursor.p;
     h.T = c->T;
-    h.icol = inter_cols(c);
+    h.icol16 = inter_cols16(c);
     h.ival = inter_vals(c);
     // B's pairs addressed by 32-bit byte offsets while they fit in 4 GiB
     const dim3 grid(persistent_grid(c, pl.nunits, 6));
@@ -2077,8 +2119,11 @@ int launch_expand_rows(spg_context* c) {
     g.vrow_row = (const int*)c->vrow_row.p;
     g.vstart = (const int*)c->vstart.p;
     g.vbase = (const long long*)c->vbase.p;
+    g.vwid = (const int*)c->vwid.p;
     g.T = c->T;
+    g.W = c->W;
     g.icol = inter_cols(c);
+    g.icol16 = inter_cols16(c);
     g.ival = inter_vals(c);
     const int* vl = (const int*)c->vlist.p;
     // untiled rows: the slot of every entry's first product
@@ -2089,7 +2134,7 @@ int launch_expand_rows(spg_context* c) {
         SPG_TRY(fill<long long>(c, (long long*)c->ebase.p, -1LL, nent));
         hipLaunchKernelGGL(k_expand_entry_base, dim3((c->m + kBlock - 1) / kBlock), dim3(kBlock), 0,
                            c->stream, c->m, g.rptr, g.rk, g.b_ptr, g.vstart, g.vbase,
-                           (long long*)c->ebase.p);
+                           g.vwid, g.W, (long long*)c->ebase.p);
         SPG_HIP(hipGetLastError());
     }
     SPG_TRY(launch_rowchunks(c, pl));
@@ -2101,6 +2146,7 @@ int launch_expand_rows(spg_context* c) {
         h.bpair = g.bpair;
         h.ebase = (const long long*)c->ebase.p;
         h.icol = g.icol;
+        h.icol16 = g.icol16;
         h.ival = g.ival;
         const long long nunits = (nent + kBlock - 1) / kBlock;
         hipLaunchKernelGGL(k_expand_entrychunks, dim3(persistent_grid(c, nunits, 8)), dim3(kBlock), 0,
@@ -2252,8 +2298,8 @@ __device__ __forceinline__ T block_exscan_n(T x, T* total, T* sh) {
 
 // Symbolic count of tile-width virtual rows: one wave per row with a
 // wave-private bitmap of the tile's columns (no workgroup barriers, so the
-// waves of a CU overlap their load latencies).  Products are read 16 bytes
-// per lane (column halves of the product groups), the next row's
+// waves of a CU overlap their load latencies).  Products are read four
+// 16-bit columns (8 bytes) per lane and load, the next row's
 // descriptor is loaded while the current one is counted, and the finished
 // bitmap is stored for the merge pass when the row is larger than the small
 // tile class.
@@ -2261,7 +2307,7 @@ constexpr int kSymWaves = kPT / 64;
 
 __global__ __launch_bounds__(kPT) void k_symbolic_tile(
     const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
-    const int* __restrict__ vlo, const int* __restrict__ icol, long long* __restrict__ vcnt,
+    const int* __restrict__ vlo, const uint16_t* __restrict__ icol16, long long* __restrict__ vcnt,
     unsigned* __restrict__ tilebits) {
     __shared__ unsigned bitsall[kSymWaves][kTileWords];
     const int wv = threadIdx.x >> 6, lane = lane_id();
@@ -2273,36 +2319,37 @@ __global__ __launch_bounds__(kPT) void k_symbolic_tile(
     int v = vlist[j];
     long long s0 = vbase[v], s1 = vbase[v + 1];
     int lo = vlo[v];
-    // 16-byte groups past the row's end are clamped to the group holding its
-    // last product (rows of the class are non-empty; the intermediates are
-    // padded to whole groups) and masked where used, so no load sits under a
-    // lane-divergent branch; the next groups are in flight while the
-    // current ones are applied
-    int4 q[2];
+    // 8-byte groups of four 16-bit columns past the row's end are clamped to
+    // the group holding its last product (rows of the class are non-empty;
+    // the column array is padded) and masked where used, so no load sits
+    // under a lane-divergent branch; the next groups are in flight while
+    // the current ones are applied
+    uint2 q[2];
 #pragma unroll
     for (int u = 0; u < 2; ++u)
-        q[u] = *(const int4*)(icol + islot(min((s0 & ~3LL) + (long long)(lane + u * 64) * 4,
-                                               (s1 - 1) & ~3LL)));
+        q[u] = *(const uint2*)(icol16 + min((s0 & ~3LL) + (long long)(lane + u * 64) * 4,
+                                            (s1 - 1) & ~3LL));
     for (;;) {
         const long long jn = j + stride;
         const int vn = jn < nlist ? vlist[jn] : -1;
         for (long long sb = s0 & ~3LL;;) {
             const long long nsb = sb + 2 * 4 * 64;
-            int4 qn[2] = {q[0], q[1]};
+            uint2 qn[2] = {q[0], q[1]};
             if (nsb < s1) {  // uniform
 #pragma unroll
                 for (int u = 0; u < 2; ++u)
-                    qn[u] = *(const int4*)(icol + islot(min(nsb + (long long)(lane + u * 64) * 4,
-                                                            (s1 - 1) & ~3LL)));
+                    qn[u] = *(const uint2*)(icol16 + min(nsb + (long long)(lane + u * 64) * 4,
+                                                         (s1 - 1) & ~3LL));
             }
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
                 const long long base = sb + (long long)(lane + u * 64) * 4;
-                const int cc[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+                int cc[4];
+                rel4(q[u], lo, cc);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     if (base + e >= s0 && base + e < s1) {
-                        const int c = cc[e] - lo;
+                        const int c = cc[e];
                         atomicOr(&bits[c >> 5], 1u << (c & 31));
                     }
                 }
@@ -2320,8 +2367,8 @@ __global__ __launch_bounds__(kPT) void k_symbolic_tile(
             lon = vlo[vn];
 #pragma unroll
             for (int u = 0; u < 2; ++u)
-                q[u] = *(const int4*)(icol + islot(min((s0n & ~3LL) + (long long)(lane + u * 64) * 4,
-                                                       (s1n - 1) & ~3LL)));
+                q[u] = *(const uint2*)(icol16 + min((s0n & ~3LL) + (long long)(lane + u * 64) * 4,
+                                                    (s1n - 1) & ~3LL));
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -2512,7 +2559,7 @@ int launch_symbolic(spg_context* c) {
                                dim3(persistent_grid(c, (cnt + kSymWaves - 1) / kSymWaves, 4)),
                                dim3(kPT), 0, c->stream, list, cnt, (const long long*)c->vbase.p,
                                (const int*)c->vlo.p,
-                               (const int*)inter_cols(c), (long long*)c->vcnt.p,
+                               (const uint16_t*)inter_cols16(c), (long long*)c->vcnt.p,
                                (unsigned*)c->tilebits.p);
         } else {
             const dim3 gw(persistent_grid(c, (cnt + kHashWaveWPB - 1) / kHashWaveWPB, 4));
@@ -2695,7 +2742,7 @@ template <bool BUILD, bool STAMP>
 __global__ __launch_bounds__(kPT) void k_merge_tile(
     const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
     const int* __restrict__ vlo, const int* __restrict__ vwid,
-    const unsigned* __restrict__ tilebits, const int* __restrict__ icol,
+    const unsigned* __restrict__ tilebits, const uint16_t* __restrict__ icol16,
     const float* __restrict__ ival, const long long* __restrict__ vout, int* __restrict__ ccol,
     float* __restrict__ cval, long long* __restrict__ cnt_out, const Done dn) {
     constexpr int WMAX = 1 << kTileMaxLog2;
@@ -2728,19 +2775,19 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
         lon = vlo[vn];
         nwn = (vwid[vn] + 31) >> 5;
     }
-    // rows of the class are non-empty; a 16-byte group past the row's end
-    // is clamped to the group holding its last product (the intermediates
-    // are padded to whole groups), and its lanes are masked where used
-    // D groups of four products per thread and pass (2 * 16 KB in flight per
-    // workgroup while the previous pass is applied)
+    // rows of the class are non-empty; a group of four products past the
+    // row's end is clamped to the group holding its last product (the
+    // intermediates are padded to whole groups), and its lanes are masked
+    // where used.  D groups of four products per thread and pass (2 * 12 KB
+    // in flight per workgroup while the previous pass is applied)
     constexpr int D = 2;
-    int4 qc[D];
+    uint2 qc[D];
     float4 qv[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         const long long b0 =
             min((s0 & ~3LL) + (long long)(d * kPT + threadIdx.x) * 4, (s1 - 1) & ~3LL);
-        qc[d] = *(const int4*)(icol + islot(b0));
+        qc[d] = *(const uint2*)(icol16 + b0);
         qv[d] = *(const float4*)(ival + islot(b0));
     }
     for (;;) {
@@ -2750,7 +2797,7 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
         if (!BUILD && threadIdx.x < nw) wb = tilebits[(long long)v * kTileWords + threadIdx.x];
         for (long long sb = s0 & ~3LL;;) {
             const long long nsb = sb + D * 4 * kPT;
-            int4 nc[D];
+            uint2 nc[D];
             float4 nv[D];
 #pragma unroll
             for (int d = 0; d < D; ++d) {
@@ -2762,19 +2809,20 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
                 for (int d = 0; d < D; ++d) {
                     const long long nb =
                         min(nsb + (long long)(d * kPT + threadIdx.x) * 4, (s1 - 1) & ~3LL);
-                    nc[d] = *(const int4*)(icol + islot(nb));
+                    nc[d] = *(const uint2*)(icol16 + nb);
                     nv[d] = *(const float4*)(ival + islot(nb));
                 }
             }
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 const long long bb = sb + (long long)(d * kPT + threadIdx.x) * 4;
-                const int cc[4] = {qc[d].x, qc[d].y, qc[d].z, qc[d].w};
+                int cc[4];
+                rel4(qc[d], lo, cc);
                 const float xx[4] = {qv[d].x, qv[d].y, qv[d].z, qv[d].w};
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     if (bb + e >= s0 && bb + e < s1) {
-                        const int c = cc[e] - lo;
+                        const int c = cc[e];
                         if constexpr (BUILD) atomicOr(&bits[c >> 5], 1u << (c & 31));
                         unsafeAtomicAdd(&acc[c], (double)xx[e]);
                     }
@@ -2797,7 +2845,7 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
             for (int d = 0; d < D; ++d) {
                 const long long nb = min((s0n & ~3LL) + (long long)(d * kPT + threadIdx.x) * 4,
                                          (s1n - 1) & ~3LL);
-                qc[d] = *(const int4*)(icol + islot(nb));
+                qc[d] = *(const uint2*)(icol16 + nb);
                 qv[d] = *(const float4*)(ival + islot(nb));
             }
         }
@@ -2876,7 +2924,7 @@ template <int CAP, int WPB, bool BUILD, bool STAMP>
 __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
     const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
     const int* __restrict__ vlo, const unsigned* __restrict__ tilebits,
-    const int* __restrict__ icol, const float* __restrict__ ival,
+    const uint16_t* __restrict__ icol16, const float* __restrict__ ival,
     const long long* __restrict__ vout, int* __restrict__ ccol, float* __restrict__ cval,
     long long* __restrict__ cnt_out, const Done dn) {
     __shared__ double accall[WPB][CAP];
@@ -2896,14 +2944,14 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
         int cnt = (int)min((long long)CAP, vout[v + 1] - o0);
         const int lo = vlo[v];
         // the first products of the accumulation pass, in flight from here:
-        // 16-byte groups past the row's end are clamped to the group holding
-        // its last product (rows of the class are non-empty; the
+        // groups of four products past the row's end are clamped to the group
+        // holding its last product (rows of the class are non-empty; the
         // intermediates are padded to whole groups) and masked where used,
         // so no load sits under a lane-divergent branch; later the next
         // group is in flight while the current one is applied
         const long long glast = (s1 - 1) & ~3LL;
         long long ga = min((s0 & ~3LL) + (long long)lane * 4, glast);
-        int4 qc = *(const int4*)(icol + islot(ga));
+        uint2 qc = *(const uint2*)(icol16 + ga);
         float4 qv = *(const float4*)(ival + islot(ga));
         uint4 wq;
         if constexpr (BUILD) {
@@ -2912,12 +2960,12 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
             __builtin_amdgcn_wave_barrier();
             for (long long sb = s0 & ~3LL; sb < s1; sb += 4 * 64) {
                 const long long base = sb + (long long)lane * 4;
-                const int4 q = *(const int4*)(icol + islot(min(base, (s1 - 1) & ~3LL)));
-                const int cc[4] = {q.x, q.y, q.z, q.w};
+                int cc[4];
+                rel4(*(const uint2*)(icol16 + min(base, (s1 - 1) & ~3LL)), lo, cc);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     if (base + e >= s0 && base + e < s1) {
-                        const int c = cc[e] - lo;
+                        const int c = cc[e];
                         atomicOr(&bits[c >> 5], 1u << (c & 31));
                     }
                 }
@@ -2954,24 +3002,25 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
         for (long long sb = s0 & ~3LL;;) {
             const long long base = sb + (long long)lane * 4;
             const long long nsb = sb + 4 * 64;
-            int4 nc = qc;
+            uint2 nc = qc;
             float4 nv = qv;
             if (nsb < s1) {  // uniform
                 ga = min(nsb + (long long)lane * 4, glast);
-                nc = *(const int4*)(icol + islot(ga));
+                nc = *(const uint2*)(icol16 + ga);
                 nv = *(const float4*)(ival + islot(ga));
             }
-            const int cc[4] = {qc.x, qc.y, qc.z, qc.w};
+            int cc[4];
+            rel4(qc, lo, cc);
             const float xx[4] = {qv.x, qv.y, qv.z, qv.w};
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 if (base + e >= s0 && base + e < s1) {
-                    const int c = cc[e] - lo;
+                    const int c = cc[e];
                     const int w = c >> 5;
                     const int r = pre[w] + __popc(bits[w] & ((1u << (c & 31)) - 1u));
                     if (r < cnt) {
                         unsafeAtomicAdd(&acc[r], (double)xx[e]);
-                        colr[r] = cc[e];
+                        colr[r] = lo + c;
                     }
                 }
             }
@@ -3354,36 +3403,37 @@ static int launch_merge_class(spg_context* c, int b, const int* list, long long 
     const long long* vbase = (const long long*)c->vbase.p;
     const int* vlo = (const int*)c->vlo.p;
     const int* icol = inter_cols(c);
+    const uint16_t* icol16 = inter_cols16(c);
     const float* ival = inter_vals(c);
     const unsigned* tb = (const unsigned*)c->tilebits.p;
     if (b == kClassTileS) {
         const dim3 g(persistent_grid(c, (cnt + 7) / 8, 4));
         if (build)
             hipLaunchKernelGGL((k_merge_tile_wave<256, 8, true, STAMP>), g, dim3(512), 0, c->stream, list,
-                               cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
+                               cnt, vbase, vlo, tb, icol16, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
         else
             hipLaunchKernelGGL((k_merge_tile_wave<256, 8, false, STAMP>), g, dim3(512), 0, c->stream, list,
-                               cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
+                               cnt, vbase, vlo, tb, icol16, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
     } else if (b == kClassTileM) {
         // one wave per workgroup: 13.8 KB of LDS each, so a CU holds 11 of
         // them (four-wave workgroups fit only twice: 8 waves)
         const dim3 g(persistent_grid(c, cnt, 11));
         if (build)
             hipLaunchKernelGGL((k_merge_tile_wave<1024, 1, true, STAMP>), g, dim3(64), 0, c->stream, list,
-                               cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
+                               cnt, vbase, vlo, tb, icol16, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
         else
             hipLaunchKernelGGL((k_merge_tile_wave<1024, 1, false, STAMP>), g, dim3(64), 0, c->stream,
-                               list, cnt, vbase, vlo, tb, icol, ival, o.obase, o.ocol, o.oval,
+                               list, cnt, vbase, vlo, tb, icol16, ival, o.obase, o.ocol, o.oval,
                                o.cnt, o.dn);
     } else if (b == kClassTile) {
         const dim3 g(persistent_grid(c, cnt, 2));
         if (build)
             hipLaunchKernelGGL((k_merge_tile<true, STAMP>), g, dim3(kPT), 0, c->stream, list, cnt, vbase,
-                               vlo, (const int*)c->vwid.p, tb, icol, ival, o.obase, o.ocol,
+                               vlo, (const int*)c->vwid.p, tb, icol16, ival, o.obase, o.ocol,
                                o.oval, o.cnt, o.dn);
         else
             hipLaunchKernelGGL((k_merge_tile<false, STAMP>), g, dim3(kPT), 0, c->stream, list, cnt, vbase,
-                               vlo, (const int*)c->vwid.p, tb, icol, ival, o.obase, o.ocol,
+                               vlo, (const int*)c->vwid.p, tb, icol16, ival, o.obase, o.ocol,
                                o.oval, o.cnt, o.dn);
     } else if (b == kClassRankW) {
         // rows per wave: 64 when the list fills 7 waves per CU that many times

@@ -173,6 +173,20 @@ inline size_t inter_words(long long np) {
 }
 inline int* inter_cols(spg_context* c) { return (int*)c->inter.p; }
 inline float* inter_vals(spg_context* c) { return (float*)c->inter.p + kIGroup; }
+// Narrow virtual rows (width <= the tile width W, so every column lies
+// within 2^13 of the row's origin lo) keep their products' columns as the
+// low 16 bits, product s at element s of a separate array placed after the
+// grouped words (64 products per 128-byte line); their values stay in the
+// grouped layout and the column half of their groups is never touched.  The
+// column is lo + ((c16 - lo) & 0xFFFF).  Elements past nprod pad the 8-byte
+// loads of a row's last four products.
+inline size_t inter16_elems(long long np) { return (size_t)np + 256; }
+inline size_t inter_bytes(long long np) {
+    return sizeof(int32_t) * inter_words(np) + sizeof(uint16_t) * inter16_elems(np);
+}
+inline uint16_t* inter_cols16(spg_context* c) {
+    return (uint16_t*)((int32_t*)c->inter.p + inter_words(c->nprod));
+}
 }  // namespace spg
 
 namespace spg {
