@@ -2919,7 +2919,14 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
 // are accumulated in FP64 at acc[rank], the column is recorded at
 // col[rank], and the row is written out with fully coalesced stores.  With
 // BUILD, cnt_out receives the row's distinct-column count (which may exceed
-// its output capacity; only the capacity is written).
+// its output capacity; only the capacity is written).  Rows are
+// software-pipelined: the next row's descriptor is loaded while the current
+// one is accumulated, and its first 256 products (and its stored bitmap)
+// while the current one is emitted, so a row of the small class costs about
+// one memory round trip; a row whose products fit one pass builds its
+// bitmap from the products already in registers.
+constexpr int kTileSWPB = 3;  // waves per workgroup of the small tile class
+
 template <int CAP, int WPB, bool BUILD, bool STAMP>
 __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
     const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
@@ -2937,31 +2944,52 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
     unsigned* bits = bitsall[wv];
     unsigned short* pre = preall[wv];
     const long long stride = (long long)gridDim.x * WPB;
-    for (long long j = (long long)blockIdx.x * WPB + wv; j < nlist; j += stride) {
-        const int v = vlist[j];
-        const long long s0 = vbase[v], s1 = vbase[v + 1];
-        const long long o0 = vout[v];
-        int cnt = (int)min((long long)CAP, vout[v + 1] - o0);
-        const int lo = vlo[v];
-        // the first products of the accumulation pass, in flight from here:
-        // groups of four products past the row's end are clamped to the group
-        // holding its last product (rows of the class are non-empty; the
-        // intermediates are padded to whole groups) and masked where used,
-        // so no load sits under a lane-divergent branch; later the next
-        // group is in flight while the current one is applied
-        const long long glast = (s1 - 1) & ~3LL;
-        long long ga = min((s0 & ~3LL) + (long long)lane * 4, glast);
-        uint2 qc = *(const uint2*)(icol16 + ga);
-        float4 qv = *(const float4*)(ival + islot(ga));
-        uint4 wq;
+    long long j = (long long)blockIdx.x * WPB + wv;
+    if (j >= nlist) return;
+    // row j: descriptor, first products and (stored) bitmap.  Groups of four
+    // products past the row's end are clamped to the group holding its last
+    // product (rows of the class are non-empty; the intermediates are padded
+    // to whole groups) and masked where used, so no load sits under a
+    // lane-divergent branch.
+    int v = vlist[j];
+    long long s0 = vbase[v], s1 = vbase[v + 1], o0 = vout[v], o1 = vout[v + 1];
+    int lo = vlo[v];
+    uint2 qc;
+    float4 qv;
+    uint4 wq = make_uint4(0u, 0u, 0u, 0u);
+    {
+        const long long ga = min((s0 & ~3LL) + (long long)lane * 4, (s1 - 1) & ~3LL);
+        qc = *(const uint2*)(icol16 + ga);
+        qv = *(const float4*)(ival + islot(ga));
+        if constexpr (!BUILD) wq = *(const uint4*)(tilebits + (long long)v * kTileWords + lane * 4);
+    }
+    long long jn = j + stride;
+    int vn = jn < nlist ? vlist[jn] : -1;
+    for (;;) {
+        // the next row's descriptor, in flight during this row
+        long long s0n = 0, s1n = 0, o0n = 0, o1n = 0;
+        int lon = 0;
+        if (vn >= 0) {
+            s0n = vbase[vn];
+            s1n = vbase[vn + 1];
+            o0n = vout[vn];
+            o1n = vout[vn + 1];
+            lon = vlo[vn];
+        }
+        const long long jnn = jn + stride;
+        const int vnn = vn >= 0 && jnn < nlist ? vlist[jnn] : -1;
+        const long long sfirst = s0 & ~3LL, glast = (s1 - 1) & ~3LL;
+        int cnt = (int)min((long long)CAP, o1 - o0);
         if constexpr (BUILD) {
             *(uint4*)(bits + lane * 4) = make_uint4(0u, 0u, 0u, 0u);
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
-            for (long long sb = s0 & ~3LL; sb < s1; sb += 4 * 64) {
+            for (long long sb = sfirst; sb < s1; sb += 4 * 64) {
                 const long long base = sb + (long long)lane * 4;
+                uint2 q = qc;  // the first pass: the products already loaded
+                if (sb != sfirst) q = *(const uint2*)(icol16 + min(base, glast));  // uniform
                 int cc[4];
-                rel4(*(const uint2*)(icol16 + min(base, (s1 - 1) & ~3LL)), lo, cc);
+                rel4(q, lo, cc);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     if (base + e >= s0 && base + e < s1) {
@@ -2973,8 +3001,6 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
             wq = *(const uint4*)(bits + lane * 4);
-        } else {
-            wq = *(const uint4*)(tilebits + (long long)v * kTileWords + lane * 4);
         }
         const int p0 = __popc(wq.x), p1 = __popc(wq.y), p2 = __popc(wq.z), p3 = __popc(wq.w);
         const int mine = p0 + p1 + p2 + p3;
@@ -2999,13 +3025,13 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
         for (int q = lane; q < cnt; q += 64) acc[q] = 0.0;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        for (long long sb = s0 & ~3LL;;) {
+        for (long long sb = sfirst;;) {
             const long long base = sb + (long long)lane * 4;
             const long long nsb = sb + 4 * 64;
             uint2 nc = qc;
             float4 nv = qv;
             if (nsb < s1) {  // uniform
-                ga = min(nsb + (long long)lane * 4, glast);
+                const long long ga = min(nsb + (long long)lane * 4, glast);
                 nc = *(const uint2*)(icol16 + ga);
                 nv = *(const float4*)(ival + islot(ga));
             }
@@ -3029,6 +3055,13 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
             sb = nsb;
             if (sb >= s1) break;
         }
+        // the next row's first products and stored bitmap, in flight during the emit
+        if (vn >= 0) {
+            const long long ga = min((s0n & ~3LL) + (long long)lane * 4, (s1n - 1) & ~3LL);
+            qc = *(const uint2*)(icol16 + ga);
+            qv = *(const float4*)(ival + islot(ga));
+            if constexpr (!BUILD) wq = *(const uint4*)(tilebits + (long long)vn * kTileWords + lane * 4);
+        }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         for (int q = lane; q < cnt; q += 64) {
@@ -3038,6 +3071,15 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
         if (STAMP && lane == 0) mark_done(dn, v);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
+        if (vn < 0) break;
+        v = vn;
+        s0 = s0n;
+        s1 = s1n;
+        o0 = o0n;
+        o1 = o1n;
+        lo = lon;
+        jn = jnn;
+        vn = vnn;
     }
 }
 
@@ -3407,13 +3449,19 @@ static int launch_merge_class(spg_context* c, int b, const int* list, long long 
     const float* ival = inter_vals(c);
     const unsigned* tb = (const unsigned*)c->tilebits.p;
     if (b == kClassTileS) {
-        const dim3 g(persistent_grid(c, (cnt + 7) / 8, 4));
+        // three waves per workgroup: 13.8 KB of LDS, so one fits beside the
+        // two dense-tile workgroups of a CU (133 KB) and the medium class's
+        // one-wave workgroup, and runs while the dense class does
+        constexpr int WPB = kTileSWPB;
+        const dim3 g(persistent_grid(c, (cnt + WPB - 1) / WPB, 8));
         if (build)
-            hipLaunchKernelGGL((k_merge_tile_wave<256, 8, true, STAMP>), g, dim3(512), 0, c->stream, list,
-                               cnt, vbase, vlo, tb, icol16, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
+            hipLaunchKernelGGL((k_merge_tile_wave<256, WPB, true, STAMP>), g, dim3(64 * WPB), 0, c->stream,
+                               list, cnt, vbase, vlo, tb, icol16, ival, o.obase, o.ocol, o.oval, o.cnt,
+                               o.dn);
         else
-            hipLaunchKernelGGL((k_merge_tile_wave<256, 8, false, STAMP>), g, dim3(512), 0, c->stream, list,
-                               cnt, vbase, vlo, tb, icol16, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
+            hipLaunchKernelGGL((k_merge_tile_wave<256, WPB, false, STAMP>), g, dim3(64 * WPB), 0, c->stream,
+                               list, cnt, vbase, vlo, tb, icol16, ival, o.obase, o.ocol, o.oval, o.cnt,
+                               o.dn);
     } else if (b == kClassTileM) {
         // one wave per workgroup: 13.8 KB of LDS each, so a CU holds 11 of
         // them (four-wave workgroups fit only twice: 8 waves)
