@@ -2869,7 +2869,14 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
         }
         if (cnt_out && threadIdx.x == 0) cnt_out[v] = tot;
         ob = uni64(ob);
+        // the wave's outputs from a scalar base, 32-bit ranks per lane; after
+        // a symbolic pass the row's range holds every column, a predicted
+        // capacity (BUILD) may cut it short
+        int* __restrict__ cw = ccol + ob;
+        float* __restrict__ vw = cval + ob;
+        const int room = BUILD ? (int)max(0LL, min((long long)INT_MAX, lim - ob)) : INT_MAX;
         const int ncol = uni(nw) * 32, wcol = wv * 1024;
+        int rb = 0;
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             if (wcol + 64 * i >= ncol) break;  // past the row's window
@@ -2879,15 +2886,15 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
             if (m == 0ull) continue;
             if ((m >> lane) & 1ull) {
                 const int col = wcol + 64 * i + lane;
-                const long long o = ob + __popcll(m & lanemask_lt());
+                const int r = rb + (int)__popcll(m & lanemask_lt());
                 const double a = acc[col];
                 acc[col] = 0.0;
-                if (o < lim) {  // slot range fixed by the symbolic pass
-                    ccol[o] = lo + col;
-                    cval[o] = (float)a;
+                if (!BUILD || r < room) {
+                    cw[r] = lo + col;
+                    vw[r] = (float)a;
                 }
             }
-            ob += __popcll(m);
+            rb += (int)__popcll(m);
         }
         __syncthreads();  // emitted: accumulators cleared, bitmap and wtot free
         if (STAMP && threadIdx.x == 0) mark_done(dn, v);
