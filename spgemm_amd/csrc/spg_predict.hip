@@ -228,7 +228,7 @@ using namespace spg;
 
 struct spg_predictor {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // the null stream
     uint64_t seed = 0;
     float mu[4] = {0, 0, 0, 0};
     float sd[4] = {1, 1, 1, 1};
@@ -318,8 +318,11 @@ int spgPredictorCreate(spg_predictor** out, int device, uint64_t seed) {
     if (!p) return SPG_ENOMEM;
     p->device = device;
     p->seed = seed;
-    if (hipSetDevice(device) != hipSuccess ||
-        hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+    // the predictor's copies and training kernels go to the device's null
+    // stream: a stream of its own would take one of the process's few
+    // hardware queues (4 by default), and a context's merge streams that
+    // end up sharing a queue lose their overlap
+    if (hipSetDevice(device) != hipSuccess) {
         delete p;
         return SPG_EHIP;
     }
@@ -346,10 +349,9 @@ int spgPredictorCreate(spg_predictor** out, int device, uint64_t seed) {
 int spgPredictorDestroy(spg_predictor* p) {
     if (!p) return SPG_EINVAL;
     (void)hipSetDevice(p->device);
-    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    (void)hipStreamSynchronize(p->stream);
     for (DevBuf* b : {&p->P, &p->M, &p->Vv, &p->sfeat, &p->starget})
         if (b->p) (void)hipFree(b->p);
-    if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
     return SPG_OK;
 }

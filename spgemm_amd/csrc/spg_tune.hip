@@ -2,9 +2,10 @@
 // counterpart: the reference fixes its bin factor, bin count and block
 // shapes at compile time, src/mm.cu:17,205,262).
 //
-// Coordinate descent from the context's current settings over tile_log2,
-// long_row_ub, item_products, expand_mode, heavy_factor and rank_min_count:
-// every value of
+// Coordinate descent over tile_log2, long_row_ub, item_products,
+// expand_mode, heavy_factor and rank_min_count, from the fastest of the
+// context's current settings and a few starting points (both expansion
+// mappings at several long-row bounds): every value of
 // one tunable is timed (wall clock of a full product or batch, median of
 // reps runs after one warm-up) with the others held, the fastest kept if it
 // beats the incumbent by more than 1 %, and the tunables are visited in an
@@ -112,6 +113,26 @@ int spgTune(spg_context* c, const spg_predictor* p, int reps, uint64_t seed,
     inc.timing = 0;
     double inc_ms = time_config(c, p, inc, reps);
     if (inc_ms < 0) return fail(c, SPG_EINVAL, "the starting configuration does not run");
+    // a few starting points of both expansion mappings and several long-row
+    // bounds (their best settings of the other tunables differ, so a descent
+    // from one of them alone can stop in the other's basin)
+    {
+        const spg_config base = inc;
+        const long long starts[][2] = {{SPG_EXPAND_ROWS, 512},  {SPG_EXPAND_ROWS, 1024},
+                                       {SPG_EXPAND_ROWS, 2048}, {SPG_EXPAND_KBINS, 512},
+                                       {SPG_EXPAND_KBINS, 1024}};
+        for (const auto& st : starts) {
+            if (st[0] == base.expand_mode && st[1] == base.long_row_ub) continue;
+            spg_config cand = base;
+            set(cand, 3, st[0]);
+            set(cand, 1, st[1]);
+            const double ms = time_config(c, p, cand, reps);
+            if (ms >= 0 && ms < inc_ms) {
+                inc = cand;
+                inc_ms = ms;
+            }
+        }
+    }
     std::vector<int> order = {0, 1, 2, 3, 4, 5};
     Xoshiro rng(seed);
     for (int q = 5; q > 0; --q) std::swap(order[(size_t)q], order[(size_t)rng.below((uint64_t)q + 1)]);
