@@ -2534,6 +2534,11 @@ static int class_offsets(spg_context* c, long long* off) {
     return readback(c, c->vclass_off.p, sizeof(long long) * (kNumVClasses + 1), off);
 }
 
+static bool aux_streams(spg_context* c);
+static int fork_aux(spg_context* c);
+static int join_aux(spg_context* c);
+constexpr long long kForkMinProducts = 1LL << 26;
+
 int launch_symbolic(spg_context* c) {
     const long long V = c->V;
     SPG_TRY(fill<long long>(c, (long long*)c->vcnt.p, 0, V));
@@ -2550,10 +2555,18 @@ int launch_symbolic(spg_context* c) {
     SPG_TRY(class_offsets(c, off));
     const int* vl = (const int*)c->vlist.p;
     tmark(c, SPG_PHASE_SYMBOLIC, 0);
-    for (int b = 0; b < kNumVClasses; ++b) {
+    // the tile class first, on the context's stream; the hash classes beside
+    // it on the two merge streams (they hold little LDS)
+    const bool par = c->nprod >= kForkMinProducts && aux_streams(c);
+    if (par) SPG_TRY(fork_aux(c));
+    const hipStream_t main = c->stream;
+    int nside = 0;
+    for (int q = 0; q < kNumVClasses; ++q) {
+        const int b = q == 0 ? kClassTile : (q <= kClassTile ? q - 1 : q);
         const long long cnt = off[b + 1] - off[b];
         if (cnt <= 0) continue;
         const int* list = vl + off[b];
+        if (par && b != kClassTile) c->stream = c->aux[nside++ & 1];
         if (b == kClassTile) {
             hipLaunchKernelGGL(k_symbolic_tile,
                                dim3(persistent_grid(c, (cnt + kSymWaves - 1) / kSymWaves, 4)),
@@ -2588,11 +2601,13 @@ int launch_symbolic(spg_context* c) {
                 case 11: launch_sym_hash<11>(c, list, cnt); break;
                 case 12: launch_sym_hash<12>(c, list, cnt); break;
                 case 13: launch_sym_hash<13>(c, list, cnt); break;
-                default: return fail(c, SPG_EINVAL, "bad symbolic class");
+                default: c->stream = main; return fail(c, SPG_EINVAL, "bad symbolic class");
             }
         }
+        c->stream = main;
         SPG_HIP(hipGetLastError());
     }
+    if (par) SPG_TRY(join_aux(c));
     tmark(c, SPG_PHASE_SYMBOLIC, 1);
     SPG_TRY(scan_i64(c, (const int64_t*)c->vcnt.p, (int64_t*)c->vout.p, V));
     return SPG_OK;
@@ -3592,7 +3607,6 @@ static int join_aux(spg_context* c) {
 // one after the other).
 // Products below this many intermediate products merge on one stream: the
 // fork and join cost tens of microseconds, more than the overlap saves.
-constexpr long long kForkMinProducts = 1LL << 26;
 constexpr long long kOverlapMaxRows = 1LL << 15;  // rank/hash classes run beside the dense one
 
 static int launch_merge_classes(spg_context* c, const int* vl, const long long* off,
