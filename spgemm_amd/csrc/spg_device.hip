@@ -2245,7 +2245,66 @@ __global__ void k_vclass_symbolic(long long V, const long long* __restrict__ vub
     long long need = 2 * min(u, (long long)width);
     int lg = kHashMinLog2;
     while ((1LL << lg) < need) ++lg;
+    // a row of a narrow window (at most 2^16 columns) with more than 256
+    // products: counted through a bitmap of its window by one wave
+    if (width <= kRankWaveWords * 32 && lg >= 10) {
+        cls[v] = kClassRankW;
+        return;
+    }
     cls[v] = lg - kHashMinLog2;  // lg <= kHashMaxLog2 since u <= long_row_ub
+}
+
+// Symbolic count of rows of a window of at most NW * 32 columns (the
+// batch's untiled rows of more than 256 products): one wave per row with a
+// private bitmap of the window (8 KB), an LDS OR per product and a
+// popcount over the window's words, which are cleared as they are counted.
+// A workgroup hash per row (2048 slots for up to 1024 products) zeroed its
+// table, probed with compare-and-swap and synchronized the workgroup twice
+// per row.
+template <int NW, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_symbolic_rank_wave(
+    const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
+    const int* __restrict__ vlo, const int* __restrict__ vwid, const int* __restrict__ icol,
+    long long* __restrict__ vcnt) {
+    static_assert(NW % 256 == 0, "four words per lane and pass");
+    __shared__ __align__(16) unsigned bitsall[WPB][NW];
+    const int wv = threadIdx.x >> 6, lane = lane_id();
+    unsigned* bits = bitsall[wv];
+    for (int w = lane * 4; w < NW; w += 256) *(uint4*)(bits + w) = make_uint4(0u, 0u, 0u, 0u);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const long long stride = (long long)gridDim.x * WPB;
+    for (long long j = (long long)blockIdx.x * WPB + wv; j < nlist; j += stride) {
+        const int v = vlist[j];
+        const long long s0 = vbase[v], s1 = vbase[v + 1];
+        const int lo = vlo[v], nw = (vwid[v] + 31) >> 5;  // <= NW (class condition)
+        // four products per lane and pass, their loads issued together (a
+        // position past the row reads its last product and is masked)
+        for (long long sb = s0; sb < s1; sb += 4 * 64) {
+            int c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) c[u] = icol[islot(min(sb + u * 64 + lane, s1 - 1))];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (sb + u * 64 + lane < s1) {
+                    const int cc = c[u] - lo;
+                    atomicOr(&bits[cc >> 5], 1u << (cc & 31));
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        int mine = 0;
+        for (int w = lane * 4; w < nw; w += 256) {  // words past nw are zero
+            const uint4 q = *(const uint4*)(bits + w);
+            mine += __popc(q.x) + __popc(q.y) + __popc(q.z) + __popc(q.w);
+            *(uint4*)(bits + w) = make_uint4(0u, 0u, 0u, 0u);
+        }
+        mine = wave_sum(mine);
+        if (lane == 0) vcnt[v] = mine;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
 }
 
 template <int LOG2H>
@@ -2567,7 +2626,13 @@ int launch_symbolic(spg_context* c) {
         if (cnt <= 0) continue;
         const int* list = vl + off[b];
         if (par && b != kClassTile) c->stream = c->aux[nside++ & 1];
-        if (b == kClassTile) {
+        if (b == kClassRankW) {
+            constexpr int WPB = 4;  // 32 KB of bitmaps per workgroup
+            hipLaunchKernelGGL((k_symbolic_rank_wave<kRankWaveWords, WPB>),
+                               dim3(persistent_grid(c, (cnt + WPB - 1) / WPB, 5)), dim3(64 * WPB), 0,
+                               c->stream, list, cnt, (const long long*)c->vbase.p, (const int*)c->vlo.p,
+                               (const int*)c->vwid.p, (const int*)inter_cols(c), (long long*)c->vcnt.p);
+        } else if (b == kClassTile) {
             hipLaunchKernelGGL(k_symbolic_tile,
                                dim3(persistent_grid(c, (cnt + kSymWaves - 1) / kSymWaves, 4)),
                                dim3(kPT), 0, c->stream, list, cnt, (const long long*)c->vbase.p,
