@@ -3016,7 +3016,8 @@ constexpr int kTileSWPB = 3;  // waves per workgroup of the small tile class
 
 template <int CAP, int WPB, bool BUILD, bool STAMP>
 __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
-    const int* __restrict__ vlist, long long nlist, const long long* __restrict__ vbase,
+    const int* __restrict__ vlist, long long nlist, unsigned long long* __restrict__ ctr,
+    const long long* __restrict__ vbase,
     const int* __restrict__ vlo, const unsigned* __restrict__ tilebits,
     const uint16_t* __restrict__ icol16, const float* __restrict__ ival,
     const long long* __restrict__ vout, int* __restrict__ ccol, float* __restrict__ cval,
@@ -3030,9 +3031,28 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
     int* colr = colall[wv];
     unsigned* bits = bitsall[wv];
     unsigned short* pre = preall[wv];
-    const long long stride = (long long)gridDim.x * WPB;
-    long long j = (long long)blockIdx.x * WPB + wv;
-    if (j >= nlist) return;
+    // Rows in chunks of R from the counter *ctr (zero at launch), so that
+    // waves that start late (while the dense class holds the LDS) take less:
+    // the wave's next chunk is claimed when it enters the current one.
+    constexpr int R = CAP <= 256 ? 16 : 8;
+    auto ticket = [&]() -> long long {
+        unsigned t = 0;
+        if (lane == 0) t = (unsigned)atomicAdd(ctr, 1ull);
+        return (long long)__builtin_amdgcn_readfirstlane(t) * R;
+    };
+    long long f = ticket();  // the last row handed out, its chunk's end, the next chunk
+    if (f >= nlist) return;
+    long long fe = min(f + R, nlist), fnext = ticket();
+    auto next_row = [&]() -> long long {  // the row after f, or -1
+        if (f < 0) return -1;
+        if (++f < fe) return f;
+        f = fnext;
+        if (f >= nlist) return f = -1;
+        fe = min(f + R, nlist);
+        fnext = ticket();
+        return f;
+    };
+    long long j = f;
     // row j: descriptor, first products and (stored) bitmap.  Groups of four
     // products past the row's end are clamped to the group holding its last
     // product (rows of the class are non-empty; the intermediates are padded
@@ -3050,8 +3070,8 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
         qv = *(const float4*)(ival + islot(ga));
         if constexpr (!BUILD) wq = *(const uint4*)(tilebits + (long long)v * kTileWords + lane * 4);
     }
-    long long jn = j + stride;
-    int vn = jn < nlist ? vlist[jn] : -1;
+    long long jn = next_row();
+    int vn = jn >= 0 ? vlist[jn] : -1;
     for (;;) {
         // the next row's descriptor, in flight during this row
         long long s0n = 0, s1n = 0, o0n = 0, o1n = 0;
@@ -3063,8 +3083,8 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
             o1n = vout[vn + 1];
             lon = vlo[vn];
         }
-        const long long jnn = jn + stride;
-        const int vnn = vn >= 0 && jnn < nlist ? vlist[jnn] : -1;
+        const long long jnn = vn >= 0 ? next_row() : -1;
+        const int vnn = jnn >= 0 ? vlist[jnn] : -1;
         const long long sfirst = s0 & ~3LL, glast = (s1 - 1) & ~3LL;
         int cnt = (int)min((long long)CAP, o1 - o0);
         if constexpr (BUILD) {
@@ -3524,6 +3544,14 @@ static void launch_merge_hash(spg_context* c, const int* list, long long cnt, co
                        (const float*)inter_vals(c), o.obase, o.ocol, o.oval, o.cnt, o.dn);
 }
 
+// A work counter of its own for one launch, zeroed on the context's current
+// stream (a fill kernel: also inside a graph capture).
+static int zeroed_counter(spg_context* c, unsigned long long** ctr) {
+    *ctr = (unsigned long long*)scratch_alloc(c, 128);
+    if (!*ctr) return fail(c, SPG_ENOMEM, "merge counter");
+    return fill<unsigned long long>(c, *ctr, 0ull, 1);
+}
+
 // One merge kernel over the virtual rows list[0..cnt) of class b.  build:
 // tile rows build their column bitmap themselves (no symbolic pass ran).
 template <bool STAMP>
@@ -3541,24 +3569,29 @@ static int launch_merge_class(spg_context* c, int b, const int* list, long long 
         // one-wave workgroup, and runs while the dense class does
         constexpr int WPB = kTileSWPB;
         const dim3 g(persistent_grid(c, (cnt + WPB - 1) / WPB, 8));
+        unsigned long long* ctr = nullptr;
+        SPG_TRY(zeroed_counter(c, &ctr));
         if (build)
             hipLaunchKernelGGL((k_merge_tile_wave<256, WPB, true, STAMP>), g, dim3(64 * WPB), 0, c->stream,
-                               list, cnt, vbase, vlo, tb, icol16, ival, o.obase, o.ocol, o.oval, o.cnt,
-                               o.dn);
+                               list, cnt, ctr, vbase, vlo, tb, icol16, ival, o.obase, o.ocol, o.oval,
+                               o.cnt, o.dn);
         else
             hipLaunchKernelGGL((k_merge_tile_wave<256, WPB, false, STAMP>), g, dim3(64 * WPB), 0, c->stream,
-                               list, cnt, vbase, vlo, tb, icol16, ival, o.obase, o.ocol, o.oval, o.cnt,
-                               o.dn);
+                               list, cnt, ctr, vbase, vlo, tb, icol16, ival, o.obase, o.ocol, o.oval,
+                               o.cnt, o.dn);
     } else if (b == kClassTileM) {
         // one wave per workgroup: 13.8 KB of LDS each, so a CU holds 11 of
         // them (four-wave workgroups fit only twice: 8 waves)
         const dim3 g(persistent_grid(c, cnt, 11));
+        unsigned long long* ctr = nullptr;
+        SPG_TRY(zeroed_counter(c, &ctr));
         if (build)
             hipLaunchKernelGGL((k_merge_tile_wave<1024, 1, true, STAMP>), g, dim3(64), 0, c->stream, list,
-                               cnt, vbase, vlo, tb, icol16, ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
+                               cnt, ctr, vbase, vlo, tb, icol16, ival, o.obase, o.ocol, o.oval, o.cnt,
+                               o.dn);
         else
             hipLaunchKernelGGL((k_merge_tile_wave<1024, 1, false, STAMP>), g, dim3(64), 0, c->stream,
-                               list, cnt, vbase, vlo, tb, icol16, ival, o.obase, o.ocol, o.oval,
+                               list, cnt, ctr, vbase, vlo, tb, icol16, ival, o.obase, o.ocol, o.oval,
                                o.cnt, o.dn);
     } else if (b == kClassTile) {
         const dim3 g(persistent_grid(c, cnt, 2));
