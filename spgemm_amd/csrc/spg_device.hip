@@ -3366,13 +3366,17 @@ __device__ __forceinline__ void rank_wave_load(const int* __restrict__ icol,
     }
 }
 
-// Rank of window column cc: set columns before it (pair prefix + the
-// words of its pair before it + the bits of its word below it).
+// Rank of window column cc: set columns before it (quad prefix + the
+// words of its quad before it + the bits of its word below it).
 __device__ __forceinline__ int rank_of(const unsigned* bits, const unsigned short* pre, int cc) {
-    const int w = cc >> 5;
-    const uint2 bb = *(const uint2*)(bits + (w & ~1));
+    const int w = cc >> 5, k = w & 3;
+    const uint4 q = *(const uint4*)(bits + (w & ~3));
     const unsigned low = (1u << (cc & 31)) - 1u;
-    return (int)pre[w >> 1] + ((w & 1) ? __popc(bb.x) + __popc(bb.y & low) : __popc(bb.x & low));
+    const unsigned a = k == 0 ? q.x & low : q.x;
+    const unsigned b = k == 1 ? q.y & low : (k > 1 ? q.y : 0u);
+    const unsigned c = k == 2 ? q.z & low : (k > 2 ? q.z : 0u);
+    const unsigned d = k == 3 ? q.w & low : 0u;
+    return (int)pre[w >> 2] + __popc(a) + __popc(b) + __popc(c) + __popc(d);
 }
 
 template <int NW, int CAP, bool STAMP>
@@ -3382,9 +3386,10 @@ __global__ __launch_bounds__(64) void k_merge_rank_wave(
     const float* __restrict__ ival, const long long* __restrict__ vout, int* __restrict__ ccol,
     float* __restrict__ cval, long long* __restrict__ cnt_out, const Done dn) {
     constexpr int PER = kRankWavePer;
-    static_assert(NW % 2 == 0, "words are handled in pairs");
-    __shared__ __align__(8) unsigned bits[NW];
-    __shared__ unsigned short pre[NW / 2];  // rank before each word pair
+    constexpr int NQ = NW / 256;  // 16-byte word quads per lane, at most
+    static_assert(NW % 256 == 0, "words are handled in quads, the same number per lane");
+    __shared__ __align__(16) unsigned bits[NW];
+    __shared__ unsigned short pre[NW / 4];  // rank before each word quad
     __shared__ double acc[CAP];
     __shared__ int colr[CAP];
     const int lane = lane_id();
@@ -3442,14 +3447,18 @@ __global__ __launch_bounds__(64) void k_merge_rank_wave(
             for (int q = lane; q < cnt; q += 64) acc[q] = 0.0;
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
-            // prefix popcounts over word pairs, an even number of words per
-            // lane (words past nw are zero)
-            const int wpt = (((nw + 63) >> 6) + 1) & ~1;
-            const int w0 = lane * wpt, w1 = min(w0 + wpt, nw);
+            // prefix popcounts over word quads, a multiple of four words per
+            // lane (words past nw are zero; 64 * wpt <= NW), held in
+            // registers from the count to the prefix
+            const int wpt = (((nw + 63) >> 6) + 3) & ~3;
+            const int w0 = lane * wpt, nq = max(0, min(wpt, nw - w0) + 3) >> 2;
+            uint4 qw[NQ];
             int mine = 0;
-            for (int w = w0; w < w1; w += 2) {
-                const uint2 bb = *(const uint2*)(bits + w);
-                mine += __popc(bb.x) + __popc(bb.y);
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                qw[i] = make_uint4(0u, 0u, 0u, 0u);
+                if (i < nq) qw[i] = *(const uint4*)(bits + w0 + 4 * i);
+                mine += __popc(qw[i].x) + __popc(qw[i].y) + __popc(qw[i].z) + __popc(qw[i].w);
             }
             int inc = mine;
 #pragma unroll
@@ -3460,10 +3469,10 @@ __global__ __launch_bounds__(64) void k_merge_rank_wave(
             const int total = __shfl(inc, 63, 64);
             if (cnt_out && lane == 0) cnt_out[v] = total;
             int run = inc - mine;
-            for (int w = w0; w < w1; w += 2) {
-                const uint2 bb = *(const uint2*)(bits + w);
-                pre[w >> 1] = (unsigned short)run;
-                run += __popc(bb.x) + __popc(bb.y);
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                if (i < nq) pre[(w0 >> 2) + i] = (unsigned short)run;
+                run += __popc(qw[i].x) + __popc(qw[i].y) + __popc(qw[i].z) + __popc(qw[i].w);
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -3509,7 +3518,9 @@ __global__ __launch_bounds__(64) void k_merge_rank_wave(
                 cval[o0 + q] = (float)acc[q];
             }
             if (STAMP && lane == 0) mark_done(dn, v);
-            for (int w = w0; w < w1; ++w) bits[w] = 0u;  // a lane clears the words it counted
+#pragma unroll
+            for (int i = 0; i < NQ; ++i)  // a lane clears the words it counted
+                if (i < nq) *(uint4*)(bits + w0 + 4 * i) = make_uint4(0u, 0u, 0u, 0u);
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
         }

@@ -374,3 +374,32 @@ def test_wide_row_merge_classes(spg, orc, gpu_ctx, rank_min):
                   cfg=dict(long_row_ub=4096, rank_min_count=rank_min))
     assert int(np.diff(out["C"][0]).max()) > 3000 and int(np.diff(out["C"][0]).min()) < 250
     compare(out, oracle_all(orc, A, (m, K), B, (K, n)))
+
+
+@pytest.mark.parametrize("n", [8200, 40001, 65536])
+def test_wave_rank_windows(spg, orc, gpu_ctx, n):
+    """Untiled rows of a window wider than a tile and at most 2^16 columns,
+    129 to 1024 distinct columns: the one-wave bitmap rank merge (and the
+    one-wave bitmap symbolic count).  The window widths give bitmaps of 257
+    words (one word past a quad), 1251 and 2048 words, so a lane's share
+    of words is partly or wholly past the window."""
+    print(f"case wave rank windows n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    K, per_b = 3000, 64
+    sizes = [3, 5, 8, 12, 16]  # entries per row: 192 .. 1024 products
+    m = 5 * 60
+    ra, ca = [], []
+    for i in range(m):
+        r = sizes[i % len(sizes)]
+        ra += [i] * r
+        ca += list(rng.choice(K, r, replace=False))
+    ra, ca = np.array(ra, np.int32), np.array(ca, np.int32)
+    rb = np.repeat(np.arange(K), per_b).astype(np.int32)
+    cb = np.concatenate([rng.choice(n, per_b, replace=False) for _ in range(K)]).astype(np.int32)
+    A = (ra, ca, rng.uniform(-1, 1, len(ra)).astype(np.float32))
+    B = (rb, cb, rng.uniform(-1, 1, len(rb)).astype(np.float32))
+    out = run_gpu(spg, gpu_ctx, A, (m, K), B, (K, n), via_coo=False,
+                  cfg=dict(long_row_ub=4096, rank_min_count=128))
+    cnt = np.diff(out["C"][0])
+    assert cnt.max() > 700 and cnt.min() > 128 and cnt.max() <= 1024
+    compare(out, oracle_all(orc, A, (m, K), B, (K, n)))
