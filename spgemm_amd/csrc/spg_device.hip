@@ -2614,14 +2614,18 @@ int launch_symbolic(spg_context* c) {
     SPG_TRY(class_offsets(c, off));
     const int* vl = (const int*)c->vlist.p;
     tmark(c, SPG_PHASE_SYMBOLIC, 0);
-    // the tile class first, on the context's stream; the hash classes beside
-    // it on the two merge streams (they hold little LDS)
+    // the tile class first, on the context's stream; the wave bitmap class
+    // and the hash classes beside it on the two merge streams (they hold
+    // little LDS), the wave bitmap class first (the largest of them)
     const bool par = c->nprod >= kForkMinProducts && aux_streams(c);
     if (par) SPG_TRY(fork_aux(c));
     const hipStream_t main = c->stream;
     int nside = 0;
-    for (int q = 0; q < kNumVClasses; ++q) {
-        const int b = q == 0 ? kClassTile : (q <= kClassTile ? q - 1 : q);
+    const int order[] = {kClassTile, kClassRankW, 0, 1, 2, 3, 4, 5, 6, 7, kClassRank, kClassTileS,
+                         kClassTileM};
+    static_assert(sizeof(order) / sizeof(order[0]) == kNumVClasses, "every class once");
+    static_assert(kNumHashClasses == 8, "hash classes 0..7");
+    for (const int b : order) {
         const long long cnt = off[b + 1] - off[b];
         if (cnt <= 0) continue;
         const int* list = vl + off[b];
@@ -3381,7 +3385,8 @@ __device__ __forceinline__ int rank_of(const unsigned* bits, const unsigned shor
 
 template <int NW, int CAP, bool STAMP>
 __global__ __launch_bounds__(64) void k_merge_rank_wave(
-    const int* __restrict__ vlist, long long nlist, int G, const long long* __restrict__ vbase,
+    const int* __restrict__ vlist, long long nlist, int G, unsigned long long* __restrict__ ctr,
+    const long long* __restrict__ vbase,
     const int* __restrict__ vlo, const int* __restrict__ vwid, const int* __restrict__ icol,
     const float* __restrict__ ival, const long long* __restrict__ vout, int* __restrict__ ccol,
     float* __restrict__ cval, long long* __restrict__ cnt_out, const Done dn) {
@@ -3397,7 +3402,13 @@ __global__ __launch_bounds__(64) void k_merge_rank_wave(
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     const long long ngrp = (nlist + G - 1) / G;
-    for (long long g = blockIdx.x; g < ngrp; g += gridDim.x) {
+    // groups of G rows from the counter *ctr (zero at launch)
+    auto ticket = [&]() -> long long {
+        unsigned t = 0;
+        if (lane == 0) t = (unsigned)atomicAdd(ctr, 1ull);
+        return (long long)__builtin_amdgcn_readfirstlane(t);
+    };
+    for (long long g = ticket(); g < ngrp; g = ticket()) {
         // descriptors of the group's rows, one per lane
         const long long jl = g * G + lane;
         int mv = 0, mcnt = 0, mlo = 0, mnw = 0;
@@ -3615,13 +3626,16 @@ static int launch_merge_class(spg_context* c, int b, const int* list, long long 
                                vlo, (const int*)c->vwid.p, tb, icol16, ival, o.obase, o.ocol,
                                o.oval, o.cnt, o.dn);
     } else if (b == kClassRankW) {
-        // rows per wave: 64 when the list fills 7 waves per CU that many times
+        // groups of up to 64 rows, about a quarter of a wave's share of the
+        // list (7 waves per CU), taken from a counter so that they balance
         const long long waves = (long long)c->num_cu * 7;
-        const int G = (int)std::min<long long>(64, std::max<long long>(1, cnt / waves));
+        const int G = (int)std::min<long long>(64, std::max<long long>(1, cnt / waves / 4));
+        unsigned long long* ctr = nullptr;
+        SPG_TRY(zeroed_counter(c, &ctr));
         hipLaunchKernelGGL((k_merge_rank_wave<kRankWaveWords, kRankWaveCap, STAMP>),
                            dim3(persistent_grid(c, (cnt + G - 1) / G, 7)), dim3(64), 0,
-                           c->stream, list, cnt, G, vbase, vlo, (const int*)c->vwid.p, icol, ival,
-                           o.obase, o.ocol, o.oval, o.cnt, o.dn);
+                           c->stream, list, cnt, G, ctr, vbase, vlo, (const int*)c->vwid.p, icol,
+                           ival, o.obase, o.ocol, o.oval, o.cnt, o.dn);
     } else if (b == kClassRank) {
         hipLaunchKernelGGL(k_merge_rank<STAMP>,
                            dim3(persistent_grid(c, (cnt + kRankChunk - 1) / kRankChunk, 2)),
