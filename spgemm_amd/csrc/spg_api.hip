@@ -490,7 +490,7 @@ int spgInitGEMM(spg_context* c, cm* C, int64_t* nprod) {
     SPG_TRY(ensure(c, c->vrow_row, sizeof(int32_t) * ((size_t)c->V + 1)));
     SPG_TRY(ensure(c, c->vlo, sizeof(int32_t) * ((size_t)c->V + 1)));
     SPG_TRY(ensure(c, c->vwid, sizeof(int32_t) * ((size_t)c->V + 1)));
-    SPG_TRY(ensure(c, c->counters, sizeof(uint64_t) * 64));
+    SPG_TRY(ensure(c, c->counters, sizeof(uint64_t) * kCtrWords));
     SPG_TRY(ensure(c, c->vub, sizeof(int64_t) * ((size_t)c->V + 1)));
     SPG_TRY(ensure(c, c->vbase, sizeof(int64_t) * ((size_t)c->V + 1)));
     SPG_TRY(ensure(c, c->cursor, sizeof(int32_t) * ((size_t)c->V + 1)));

@@ -3035,16 +3035,18 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
     int* colr = colall[wv];
     unsigned* bits = bitsall[wv];
     unsigned short* pre = preall[wv];
-    // Rows in chunks of R from the counter *ctr (zero at launch), so that
-    // waves that start late (while the dense class holds the LDS) take less:
-    // the wave's next chunk is claimed when it enters the current one.
+    // Rows in chunks of R: a wave's first chunk by its index, the others
+    // from the counter *ctr (zero at launch), so that waves that start late
+    // (while the dense class holds the LDS) take less; the wave's next chunk
+    // is claimed when it enters the current one.
     constexpr int R = CAP <= 256 ? 16 : 8;
+    const long long nwave = (long long)gridDim.x * WPB;
     auto ticket = [&]() -> long long {
         unsigned t = 0;
         if (lane == 0) t = (unsigned)atomicAdd(ctr, 1ull);
-        return (long long)__builtin_amdgcn_readfirstlane(t) * R;
+        return (nwave + (long long)__builtin_amdgcn_readfirstlane(t)) * R;
     };
-    long long f = ticket();  // the last row handed out, its chunk's end, the next chunk
+    long long f = ((long long)blockIdx.x * WPB + wv) * R;  // the last row handed out, its chunk's end, the next chunk
     if (f >= nlist) return;
     long long fe = min(f + R, nlist), fnext = ticket();
     auto next_row = [&]() -> long long {  // the row after f, or -1
@@ -3402,13 +3404,14 @@ __global__ __launch_bounds__(64) void k_merge_rank_wave(
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     const long long ngrp = (nlist + G - 1) / G;
-    // groups of G rows from the counter *ctr (zero at launch)
+    // groups of G rows: the first by the wave's index, the others from the
+    // counter *ctr (zero at launch)
     auto ticket = [&]() -> long long {
         unsigned t = 0;
         if (lane == 0) t = (unsigned)atomicAdd(ctr, 1ull);
-        return (long long)__builtin_amdgcn_readfirstlane(t);
+        return (long long)gridDim.x + (long long)__builtin_amdgcn_readfirstlane(t);
     };
-    for (long long g = ticket(); g < ngrp; g = ticket()) {
+    for (long long g = blockIdx.x; g < ngrp; g = ticket()) {
         // descriptors of the group's rows, one per lane
         const long long jl = g * G + lane;
         int mv = 0, mcnt = 0, mlo = 0, mnw = 0;
@@ -3566,12 +3569,19 @@ static void launch_merge_hash(spg_context* c, const int* list, long long cnt, co
                        (const float*)inter_vals(c), o.obase, o.ocol, o.oval, o.cnt, o.dn);
 }
 
-// A work counter of its own for one launch, zeroed on the context's current
-// stream (a fill kernel: also inside a graph capture).
+// Every counter of the context zeroed (one fill kernel, also inside a graph
+// capture) at the start of a merge phase, whose kernels then take pool slots.
+static int reset_counters(spg_context* c) {
+    c->ctr_pool = 0;
+    return fill<unsigned long long>(c, (unsigned long long*)c->counters.p, 0ull, kCtrWords);
+}
+
+// A work counter of its own for one merge kernel: a pool slot, zero since
+// the phase started.
 static int zeroed_counter(spg_context* c, unsigned long long** ctr) {
-    *ctr = (unsigned long long*)scratch_alloc(c, 128);
-    if (!*ctr) return fail(c, SPG_ENOMEM, "merge counter");
-    return fill<unsigned long long>(c, *ctr, 0ull, 1);
+    if (c->ctr_pool >= kCtrPoolSlots) return fail(c, SPG_EINVAL, "merge counters exhausted");
+    *ctr = (unsigned long long*)c->counters.p + kCtrPoolBase + kCtrPoolStride * c->ctr_pool++;
+    return SPG_OK;
 }
 
 // One merge kernel over the virtual rows list[0..cnt) of class b.  build:
@@ -3772,7 +3782,7 @@ static int launch_merge_classes(spg_context* c, const int* vl, const long long* 
 
 int launch_merge(spg_context* c) {
     const long long V = c->V;
-    SPG_TRY(fill<unsigned long long>(c, (unsigned long long*)c->counters.p, 0ull, 64));
+    SPG_TRY(reset_counters(c));
     if (V > 0) {
         hipLaunchKernelGGL(k_vclass_merge, dim3((unsigned)((V + kBlock - 1) / kBlock)),
                            dim3(kBlock), 0, c->stream, V, (const long long*)c->vcnt.p,
@@ -4152,7 +4162,7 @@ int launch_learned_merge(spg_context* c, spg_learned_stats* st) {
     const unsigned gV = persistent_grid(c, (V + kBlock - 1) / kBlock, 4);
 
     tmark(c, SPG_PHASE_MERGE, 0);
-    SPG_TRY(fill<unsigned long long>(c, ctr, 0ull, 64));
+    SPG_TRY(reset_counters(c));
     SPG_TRY(fill<long long>(c, vcnt, 0, V));
     if (V > 0) {
         // fbase doubles as the per-row tile weight sums until the fallback

@@ -21,6 +21,13 @@ constexpr int kHashMinLog2 = 6;         // smallest hash: 64 slots
 // Merge/symbolic classes of a virtual row.
 constexpr int kNumHashClasses = kHashMaxLog2 - kHashMinLog2 + 1;  // 64..8192
 constexpr int kNumVClasses = kNumHashClasses + 5;  // + tile (large), rank, tile small/medium, rank (wave)
+// The context's work counters (64-bit): slots below kCtrPoolBase have fixed
+// uses; above it, kCtrPoolSlots slots 128 bytes apart are handed to the
+// merge kernels of one merge phase, zeroed together when the phase starts.
+constexpr int kCtrWords = 1024;
+constexpr int kCtrPoolBase = 64;
+constexpr int kCtrPoolStride = 16;
+constexpr int kCtrPoolSlots = (kCtrWords - kCtrPoolBase) / kCtrPoolStride;
 
 // Grow-only device buffer.  fixed: memory the caller owns (an output of the
 // allocation-free entry point), never replaced; too small is an error.
@@ -130,6 +137,7 @@ struct spg_context {
     // synchronization.  Valid while the operands' structure, the config and
     // the predictor are unchanged.
     int plan_mode = 0;  // 0: off, 1: record, 2: replay
+    int ctr_pool = 0;   // counter slots handed out in this merge phase
     std::vector<unsigned char> plan;
     size_t plan_pos = 0;
 
