@@ -3027,12 +3027,12 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
     const long long* __restrict__ vout, int* __restrict__ ccol, float* __restrict__ cval,
     long long* __restrict__ cnt_out, const Done dn) {
     __shared__ double accall[WPB][CAP];
-    __shared__ int colall[WPB][CAP];
+    __shared__ unsigned short colall[WPB][CAP];
     __shared__ unsigned bitsall[WPB][kTileWords];
     __shared__ unsigned short preall[WPB][kTileWords];
     const int wv = threadIdx.x >> 6, lane = lane_id();
     double* acc = accall[wv];
-    int* colr = colall[wv];
+    unsigned short* colr = colall[wv];
     unsigned* bits = bitsall[wv];
     unsigned short* pre = preall[wv];
     // Rows in chunks of R: a wave's first chunk by its index, the others
@@ -3159,7 +3159,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
                     const int r = pre[w] + __popc(bits[w] & ((1u << (c & 31)) - 1u));
                     if (r < cnt) {
                         unsafeAtomicAdd(&acc[r], (double)xx[e]);
-                        colr[r] = lo + c;
+                        colr[r] = (unsigned short)c;
                     }
                 }
             }
@@ -3178,7 +3178,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         for (int q = lane; q < cnt; q += 64) {
-            ccol[o0 + q] = colr[q];
+            ccol[o0 + q] = lo + (int)colr[q];
             cval[o0 + q] = (float)acc[q];
         }
         if (STAMP && lane == 0) mark_done(dn, v);
@@ -3596,7 +3596,7 @@ static int launch_merge_class(spg_context* c, int b, const int* list, long long 
     const float* ival = inter_vals(c);
     const unsigned* tb = (const unsigned*)c->tilebits.p;
     if (b == kClassTileS) {
-        // three waves per workgroup: 13.8 KB of LDS, so one fits beside the
+        // three waves per workgroup: 12 KB of LDS, so one fits beside the
         // two dense-tile workgroups of a CU (133 KB) and the medium class's
         // one-wave workgroup, and runs while the dense class does
         constexpr int WPB = kTileSWPB;
@@ -3612,9 +3612,9 @@ static int launch_merge_class(spg_context* c, int b, const int* list, long long 
                                list, cnt, ctr, vbase, vlo, tb, icol16, ival, o.obase, o.ocol, o.oval,
                                o.cnt, o.dn);
     } else if (b == kClassTileM) {
-        // one wave per workgroup: 13.8 KB of LDS each, so a CU holds 11 of
-        // them (four-wave workgroups fit only twice: 8 waves)
-        const dim3 g(persistent_grid(c, cnt, 11));
+        // one wave per workgroup: 11.8 KB of LDS each (16-bit columns
+        // beside the FP64 accumulators), so a CU holds 13 of them
+        const dim3 g(persistent_grid(c, cnt, 13));
         unsigned long long* ctr = nullptr;
         SPG_TRY(zeroed_counter(c, &ctr));
         if (build)
