@@ -662,6 +662,8 @@ __global__ __launch_bounds__(kBlock) void k_row_tile_bound(const int* __restrict
 // atomic fill.
 constexpr int kTrMaxBuckets = 4096;
 constexpr int kTrMaxBucketLog2 = 14;  // rows per bucket <= 16384 (64 KB of LDS counters)
+constexpr long long kTrStageMin = 4096;  // mean entries per bucket from which buckets are staged
+constexpr size_t kTrStageLds = 144 << 10;  // LDS of a staged bucket workgroup, at most
 
 __global__ __launch_bounds__(kBlock) void k_tr_kidx(const int* __restrict__ a_ptr, int K,
                                                      int* __restrict__ kidx) {
@@ -701,7 +703,10 @@ __global__ __launch_bounds__(kBlock) void k_tr_count(const int* __restrict__ a_i
             if (e + u * kBlock < e1) atomicAdd(&hist[r[u] >> BL], 1);
     }
     __syncthreads();
-    for (int b = threadIdx.x; b < NB; b += kBlock) H[(long long)b * ntiles + blockIdx.x] = hist[b];
+    // tile-major (one contiguous row per tile), transposed to bucket-major
+    // for the scan: written bucket-major, each tile's counts were NB
+    // scattered 4-byte stores
+    for (int b = threadIdx.x; b < NB; b += kBlock) H[(long long)blockIdx.x * NB + b] = hist[b];
 }
 
 __global__ __launch_bounds__(kBlock) void k_tr_scatter(
@@ -709,7 +714,8 @@ __global__ __launch_bounds__(kBlock) void k_tr_scatter(
     long long nnz, int E, int BL, int NB, long long ntiles, const int* __restrict__ O,
     int* __restrict__ tr, int* __restrict__ tk, float* __restrict__ tv) {
     __shared__ int cur[kTrMaxBuckets];
-    for (int b = threadIdx.x; b < NB; b += kBlock) cur[b] = O[(long long)b * ntiles + blockIdx.x];
+    // the tile's offsets, tile-major (a contiguous row)
+    for (int b = threadIdx.x; b < NB; b += kBlock) cur[b] = O[(long long)blockIdx.x * NB + b];
     __syncthreads();
     const long long e0 = (long long)blockIdx.x * E, e1 = min(e0 + E, nnz);
     for (long long e = e0 + threadIdx.x; e < e1; e += 4 * kBlock) {
@@ -792,6 +798,102 @@ __global__ __launch_bounds__(kBlock) void k_tr_bucket(
     }
 }
 
+// k_tr_bucket for large buckets: the entries are placed in LDS (2^BL row
+// counters, then SW staged entries, sized at launch) and the bucket is
+// written front to back.  Placed straight into global memory, the entries
+// of the 512 buckets in flight (tens of MB of partly written lines) left
+// L2 before their lines were complete: 2.8 GB written for 0.44 GB on the
+// 256-product batch.  A bucket of more than SW entries places them
+// directly.
+__global__ __launch_bounds__(kBlock) void k_tr_bucket_staged(
+    const int* __restrict__ tr, const int* __restrict__ tk, const float* __restrict__ tv,
+    const int* __restrict__ O, long long ntiles, int NB, int BL, int m, int SW,
+    int* __restrict__ rptr, int* __restrict__ rk, float* __restrict__ rv) {
+    extern __shared__ int lds[];
+    __shared__ int sh[kBlock / 64];
+    const int nrow = 1 << BL;
+    int* rc = lds;
+    int2* stage = (int2*)(lds + max(nrow, 4));
+    const int b = blockIdx.x, r0 = b << BL;
+    const int p0 = O[(long long)b * ntiles], p1 = O[(long long)(b + 1) * ntiles];
+    const bool staged = p1 - p0 <= SW;  // uniform
+    for (int i = threadIdx.x; i < nrow; i += kBlock) rc[i] = 0;
+    __syncthreads();
+    for (int p = p0 + threadIdx.x; p < p1; p += 4 * kBlock) {
+        int r[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) r[u] = tr[min(p + u * kBlock, p1 - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (p + u * kBlock < p1) atomicAdd(&rc[r[u] - r0], 1);
+    }
+    __syncthreads();
+    const int per = (nrow + kBlock - 1) / kBlock;
+    const int i0 = min(nrow, threadIdx.x * per), i1 = min(nrow, i0 + per);
+    int mine = 0;
+    for (int i = i0; i < i1; ++i) mine += rc[i];
+    int tot;
+    int run = block_exscan<int>(mine, &tot, sh);
+    for (int i = i0; i < i1; ++i) {
+        const int cnt = rc[i];
+        rc[i] = run;
+        if (r0 + i < m) rptr[r0 + i] = p0 + run;
+        run += cnt;
+    }
+    if (b == NB - 1 && threadIdx.x == 0) rptr[m] = p1;
+    __syncthreads();
+    for (int p = p0 + threadIdx.x; p < p1; p += 4 * kBlock) {
+        int r[4], k[4];
+        float x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = min(p + u * kBlock, p1 - 1);
+            r[u] = tr[q];
+            k[u] = tk[q];
+            x[u] = tv[q];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (p + u * kBlock < p1) {
+                const int pos = atomicAdd(&rc[r[u] - r0], 1);
+                if (staged) {
+                    stage[pos] = make_int2(k[u], __float_as_int(x[u]));
+                } else {
+                    rk[p0 + pos] = k[u];
+                    rv[p0 + pos] = x[u];
+                }
+            }
+        }
+    }
+    if (!staged) return;
+    __syncthreads();
+    for (int q = threadIdx.x; q < p1 - p0; q += kBlock) {
+        const int2 e = stage[q];
+        rk[p0 + q] = e.x;
+        rv[p0 + q] = __int_as_float(e.y);
+    }
+}
+
+// out[c][r] = in[r][c] for an R x C matrix of ints, through 32 x 32 tiles
+// in LDS (both sides coalesced).
+__global__ __launch_bounds__(256) void k_transpose_i32(const int* __restrict__ in, int R, int C,
+                                                       int* __restrict__ out) {
+    __shared__ int tile[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+#pragma unroll
+    for (int j = 0; j < 32; j += 8) {
+        const int r = r0 + ty + j, c = c0 + tx;
+        if (r < R && c < C) tile[ty + j][tx] = in[(long long)r * C + c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 32; j += 8) {
+        const int c = c0 + ty + j, r = r0 + tx;
+        if (c < C && r < R) out[(long long)c * R + r] = tile[tx][ty + j];
+    }
+}
+
 // Bucket parameters of the transpose, or false when the rows are too many
 // (more than 4096 buckets of 16384) and the atomic transpose is used.
 static bool tr_params(int m, long long nnz, int* BL, int* NB, int* E, long long* ntiles) {
@@ -821,7 +923,7 @@ size_t transpose_scratch(const spg_context* c, int m, long long nnz) {
     int BL, NB, E;
     long long nt;
     if (!use_sort(c, nnz) || nnz <= 0 || !tr_params(m, nnz, &BL, &NB, &E, &nt)) return 0;
-    return (size_t)16 * (size_t)nnz + (size_t)8 * (size_t)NB * (size_t)nt + ((size_t)8 << 20);
+    return (size_t)16 * (size_t)nnz + (size_t)12 * (size_t)NB * (size_t)nt + ((size_t)8 << 20);
 }
 
 // Upper bounds, tiling and features of every row (A by rows is built).
@@ -853,21 +955,46 @@ static int launch_transpose(spg_context* c, bool* done) {
     float* tv = (float*)scratch_alloc(c, sizeof(float) * (size_t)nnz);
     int* H = (int*)scratch_alloc(c, sizeof(int) * (size_t)NB * (size_t)nt);
     int* O = (int*)scratch_alloc(c, sizeof(int) * ((size_t)NB * (size_t)nt + 1));
-    if (!kidx || !tr || !tk || !tv || !H || !O) return fail(c, SPG_ENOMEM, "transpose scratch");
+    int* HT = (int*)scratch_alloc(c, sizeof(int) * (size_t)NB * (size_t)nt);  // tile-major H, then O
+    if (!kidx || !tr || !tk || !tv || !H || !O || !HT)
+        return fail(c, SPG_ENOMEM, "transpose scratch");
     const long long nbk = ((long long)K * kGroup + kBlock - 1) / kBlock;  // 8 lanes per column
     hipLaunchKernelGGL(k_tr_kidx, dim3((unsigned)nbk), dim3(kBlock), 0, c->stream, c->A.ptr, K,
                        kidx);
     SPG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_tr_count, dim3((unsigned)nt), dim3(kBlock), 0, c->stream, c->A.idx, nnz,
-                       E, BL, NB, nt, H);
+                       E, BL, NB, nt, HT);
+    SPG_HIP(hipGetLastError());
+    // [tile][bucket] -> [bucket][tile] for the bucket-major scan, and back
+    const dim3 tb(256);
+    hipLaunchKernelGGL(k_transpose_i32, dim3((unsigned)((NB + 31) / 32), (unsigned)((nt + 31) / 32)),
+                       tb, 0, c->stream, (const int*)HT, (int)nt, NB, H);
     SPG_HIP(hipGetLastError());
     SPG_TRY(scan_i32(c, H, O, (long long)NB * nt));
-    hipLaunchKernelGGL(k_tr_scatter, dim3((unsigned)nt), dim3(kBlock), 0, c->stream, c->A.idx,
-                       c->A.val, (const int*)kidx, nnz, E, BL, NB, nt, (const int*)O, tr, tk, tv);
+    hipLaunchKernelGGL(k_transpose_i32, dim3((unsigned)((nt + 31) / 32), (unsigned)((NB + 31) / 32)),
+                       tb, 0, c->stream, (const int*)O, NB, (int)nt, HT);
     SPG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_tr_bucket, dim3((unsigned)NB), dim3(kBlock), 0, c->stream,
-                       (const int*)tr, (const int*)tk, (const float*)tv, (const int*)O, nt, NB, BL,
-                       m, (int*)c->arow_ptr.p, (int*)c->arow_k.p, (float*)c->arow_v.p);
+    hipLaunchKernelGGL(k_tr_scatter, dim3((unsigned)nt), dim3(kBlock), 0, c->stream, c->A.idx,
+                       c->A.val, (const int*)kidx, nnz, E, BL, NB, nt, (const int*)HT, tr, tk, tv);
+    SPG_HIP(hipGetLastError());
+    // buckets of many entries are placed in LDS and written front to back
+    const long long avg = nnz / NB;
+    const size_t rc_bytes = sizeof(int) * (size_t)std::max(1 << BL, 4);
+    int SW = 0;
+    if (avg >= kTrStageMin) {
+        SW = 16384;
+        while (SW > 1024 && rc_bytes + 8 * (size_t)SW > kTrStageLds) SW >>= 1;
+        if (rc_bytes + 8 * (size_t)SW > kTrStageLds) SW = 0;
+    }
+    if (SW > 0)
+        hipLaunchKernelGGL(k_tr_bucket_staged, dim3((unsigned)NB), dim3(kBlock),
+                           rc_bytes + 8 * (size_t)SW, c->stream, (const int*)tr, (const int*)tk,
+                           (const float*)tv, (const int*)O, nt, NB, BL, m, SW,
+                           (int*)c->arow_ptr.p, (int*)c->arow_k.p, (float*)c->arow_v.p);
+    else
+        hipLaunchKernelGGL(k_tr_bucket, dim3((unsigned)NB), dim3(kBlock), 0, c->stream,
+                           (const int*)tr, (const int*)tk, (const float*)tv, (const int*)O, nt, NB,
+                           BL, m, (int*)c->arow_ptr.p, (int*)c->arow_k.p, (float*)c->arow_v.p);
     SPG_HIP(hipGetLastError());
     *done = true;
     return SPG_OK;

@@ -190,6 +190,29 @@ def test_rows_mode_many_tiles(spg, orc, gpu_ctx):
     compare(out, oracle_all(orc, A, sh, A, sh))
 
 
+def test_transpose_staged_buckets(spg, orc, gpu_ctx):
+    """The counting-sort transpose with buckets of thousands of entries
+    (here one row per bucket: 256 rows of 4200 or 20000 entries), which are
+    placed in LDS and written front to back, or placed directly when they
+    exceed the staging area (the 20000-entry rows)."""
+    print("case transpose staged buckets", flush=True)
+    rng = np.random.default_rng(77)
+    m, K, n = 256, 1 << 16, 1000
+    rows, cols = [], []
+    for i in range(m):
+        r = 20000 if i % 37 == 5 else 4200
+        rows.append(np.full(r, i, np.int32))
+        cols.append(rng.choice(K, r, replace=False).astype(np.int32))
+    rows, cols = np.concatenate(rows), np.concatenate(cols)
+    vals = rng.uniform(-1, 1, len(rows)).astype(np.float32)
+    rb = np.arange(K, dtype=np.int32)
+    cb = rng.integers(0, n, K).astype(np.int32)
+    vb = rng.uniform(-1, 1, K).astype(np.float32)
+    out = run_gpu(spg, gpu_ctx, (rows, cols, vals), (m, K), (rb, cb, vb), (K, n), via_coo=False,
+                  cfg=dict(transpose_mode=spg.TRANSPOSE_SORT, expand_mode=spg.EXPAND_ROWS))
+    compare(out, oracle_all(orc, (rows, cols, vals), (m, K), (rb, cb, vb), (K, n)))
+
+
 @pytest.mark.parametrize("tmode", [2, 1], ids=["sort", "atomic"])
 def test_transpose_modes(spg, orc, gpu_ctx, tmode):
     """InitGEMM's A-by-rows transpose: the LDS counting sort and the
