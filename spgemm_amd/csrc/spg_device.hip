@@ -3136,8 +3136,9 @@ __global__ __launch_bounds__(kPT) void k_merge_tile(
 // workgroup barriers.  The row's column bitmap (stored by the symbolic
 // pass, or built here from the products when BUILD) and a prefix popcount
 // per word give every column its rank, i.e. its output position; products
-// are accumulated in FP64 at acc[rank], the column is recorded at
-// col[rank], and the row is written out with fully coalesced stores.  With
+// are accumulated in FP64 at acc[rank], the column's offset in the tile is
+// recorded at col[rank] (16 bits), and the row is written out with fully
+// coalesced stores.  With
 // BUILD, cnt_out receives the row's distinct-column count (which may exceed
 // its output capacity; only the capacity is written).  Rows are
 // software-pipelined: the next row's descriptor is loaded while the current
@@ -3175,7 +3176,8 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_tile_wave(
         if (lane == 0) t = (unsigned)atomicAdd(ctr, 1ull);
         return (nwave + (long long)__builtin_amdgcn_readfirstlane(t)) * R;
     };
-    long long f = ((long long)blockIdx.x * WPB + wv) * R;  // the last row handed out, its chunk's end, the next chunk
+    // f: the last row handed out; fe: the end of its chunk; fnext: the next chunk
+    long long f = ((long long)blockIdx.x * WPB + wv) * R;
     if (f >= nlist) return;
     long long fe = min(f + R, nlist), fnext = ticket();
     auto next_row = [&]() -> long long {  // the row after f, or -1
