@@ -601,11 +601,27 @@ __global__ __launch_bounds__(kBlock) void k_row_tile_bound(const int* __restrict
     const int wv = threadIdx.x >> 6, lane = lane_id();
     constexpr int NWV = kBlock / 64;
     const long long nr = *count;
-    for (long long q = blockIdx.x; q < nr; q += gridDim.x) {
-        const int i = list[q];
-        const int vs = vstart[i];
-        const int Ti = vstart[i + 1] - vs;  // the row's tiles (T + 1: split-point stride)
-        const int e0 = rptr[i], e1 = rptr[i + 1];
+    // the row's descriptor, and the next row's loaded while this one is summed
+    long long q = blockIdx.x;
+    int ni = q < nr ? list[q] : 0;
+    int nvs = 0, nvs1 = 0, ne0 = 0, ne1 = 0;
+    if (q < nr) {
+        nvs = vstart[ni];
+        nvs1 = vstart[ni + 1];
+        ne0 = rptr[ni];
+        ne1 = rptr[ni + 1];
+    }
+    for (; q < nr; q += gridDim.x) {
+        const int vs = nvs;
+        const int Ti = nvs1 - vs;  // the row's tiles (T + 1: split-point stride)
+        const int e0 = ne0, e1 = ne1;
+        if (q + gridDim.x < nr) {
+            ni = list[q + gridDim.x];
+            nvs = vstart[ni];
+            nvs1 = vstart[ni + 1];
+            ne0 = rptr[ni];
+            ne1 = rptr[ni + 1];
+        }
         // a wave covers G = 64 / P entries at a time, P = the tiles rounded
         // up to a power of two (at most 64): lane = (entry slot, tile)
         const int lp = Ti >= 64 ? 6 : 32 - __clz(Ti - 1);
