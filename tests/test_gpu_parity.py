@@ -14,7 +14,7 @@ from helpers import PROD_FIXTURES, csc_of, csr_of, load_prod, rel_err
 
 pytestmark = pytest.mark.gpu
 
-VAL_RTOL = 1e-5      # north-star tolerance vs the FP32 restated oracle
+VAL_RTOL = 1e-5      # value tolerance: 1e-5 relative to the FP32 restated oracle
 ZERO_ATOL = 1e-30    # exact cancellations: both sides must be ~0
 
 
