@@ -633,7 +633,7 @@ def main():
         P.close()
     ctx.close()
     if args.workload == "rmat_s18_ef16" and not args.no_batch:
-        # the batched half of the north-star ratio: the 256-product batch
+        # the batched half of the headline ratio: the 256-product batch
         # (learned + batched graph vs batched plain vs one product per launch)
         w = WORKLOADS["batch256"]
         b = batch_result(args, make_batch(w["count"], w["seed"]), "batch256")
