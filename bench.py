@@ -546,7 +546,7 @@ def main():
         return copy_probe(args.copy_probe_mib)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by "
-              "C row blocks across GPUs but this build runs on a single MI355X.", file=sys.stderr)
+              "C row blocks across GPUs but this library runs on a single MI355X.", file=sys.stderr)
         sys.exit(2)
     if args.mtx_list:
         sys.exit(bench_mtx_list(args))

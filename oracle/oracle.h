@@ -3,8 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (spgemm_amd/) links,
  * loads or calls this code.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py use it, and only as the checker or as the
- * timed CPU baseline.
+ * cpu_baseline leg of bench.py use it, and only as the correctness
+ * reference or as the timed CPU baseline.
  *
  * The functions restate the reference method (fast-squid/spGEMM) in plain C:
  *   src/coo.cpp:43-79    COO sort comparators and sortCOO
@@ -19,7 +19,7 @@
  * a non-empty B row, every offset and count is 64-bit.
  *
  * An independent textbook row-wise Gustavson product (FP64 products and FP64
- * sums) is the FP64 value judge.
+ * sums) is the FP64 value reference.
  */
 #ifndef SPGEMM_ORACLE_H_
 #define SPGEMM_ORACLE_H_
@@ -116,7 +116,7 @@ int64_t orc_spgemm_restated(int32_t m, int32_t n, int32_t K,
                             int64_t* nprod_out);
 
 /* Textbook row-wise Gustavson (independent of the method above) on A in
- * CSR.  prod32 = 0: FP64 products a*b and FP64 sums (the FP64 judge);
+ * CSR.  prod32 = 0: FP64 products a*b and FP64 sums (the FP64 reference);
  * prod32 = 1: FP32 products then FP64 sums rounded to FP32 (must equal the
  * restated method).  rows == NULL means every row; otherwise only the
  * listed rows, and C is the concatenation of those rows.  Values are
